@@ -1,0 +1,353 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker. Each function documents the
+reference routine it restates (see tdmd_oracle.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_BIN = HERE / "_ref" / "tdmd_standalone"
+
+_D = C.c_double
+_I = C.c_int
+_I64 = C.c_int64
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int)
+
+
+class OrcBox(C.Structure):
+    _fields_ = [("lo", _D * 3), ("hi", _D * 3), ("periodic", _I * 3)]
+
+
+class OrcCells(C.Structure):
+    _fields_ = [("ncx", _I), ("ncy", _I), ("ncz", _I), ("cs", _D * 3), ("ncells", _I),
+                ("counts", _PI), ("offsets", _PI), ("atoms", _PI)]
+
+
+class OrcNlist(C.Structure):
+    _fields_ = [("natoms", _I64), ("r_cut", _D), ("r_skin", _D), ("counts", _PI),
+                ("offsets", _PI), ("nbrs", _PI), ("total", _I64), ("build_pos", _PD)]
+
+
+class OrcSpline(C.Structure):
+    _fields_ = [("n", _I), ("dr", _D), ("rmin", _D), ("values", _PD), ("a", _PD), ("b", _PD),
+                ("c", _PD), ("d", _PD)]
+
+
+class OrcEam(C.Structure):
+    _fields_ = [("ntypes", _I), ("cutoff", _D), ("nrho", _I), ("nr", _I), ("drho", _D),
+                ("dr", _D), ("embedding", C.POINTER(OrcSpline)),
+                ("density", C.POINTER(OrcSpline)), ("phi", C.POINTER(OrcSpline))]
+
+
+class OrcNhc(C.Structure):
+    _fields_ = [("M", _I), ("t_target", _D), ("t_period", _D), ("dt", _D), ("n_dof", _I),
+                ("eta", _D * 8), ("eta_v", _D * 8), ("eta_m", _D * 8)]
+
+
+class OrcRunCfg(C.Structure):
+    _fields_ = [("use_eam", _I), ("morse", _D * 4), ("eam", C.POINTER(OrcEam)), ("dt", _D),
+                ("skin", _D), ("nsteps", _I), ("thermo_every", _I), ("rebuild_mode", _I),
+                ("rebuild_every", _I), ("nvt", _I), ("t_target", _D), ("t_period", _D),
+                ("nhc_length", _I)]
+
+
+class OrcRunOut(C.Structure):
+    _fields_ = [("rebuilds", _I), ("nthermo", _I), ("thermo", _PD), ("thermo_cap", _I),
+                ("last_pe", _D)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            subprocess.run(["make", "-C", str(HERE), "liboracle.so"], check=True,
+                           capture_output=True)
+        L = C.CDLL(str(LIB_PATH))
+        L.orc_cells_build.argtypes = [C.POINTER(OrcCells), _PD, _I64, C.POINTER(OrcBox), _D]
+        L.orc_cells_free.argtypes = [C.POINTER(OrcCells)]
+        L.orc_nlist_build.argtypes = [C.POINTER(OrcNlist), _PD, _I64, C.POINTER(OrcBox), _D, _D]
+        L.orc_nlist_needs_rebuild.argtypes = [C.POINTER(OrcNlist), _PD, _I64]
+        L.orc_nlist_free.argtypes = [C.POINTER(OrcNlist)]
+        L.orc_nlist_full.argtypes = [C.POINTER(OrcNlist), _PI, _PI, _PI]
+        L.orc_bruteforce_full.argtypes = [_PD, _I64, C.POINTER(OrcBox), _D, _PI, _PI, _PI, _I64]
+        L.orc_bruteforce_full.restype = _I64
+        L.orc_morse_pair.argtypes = [_D, _D, _D, _D, _D, _PD, _PD]
+        L.orc_morse_forces.argtypes = [_PD, _PD, _I64, C.POINTER(OrcBox), C.POINTER(OrcNlist),
+                                       _D, _D, _D, _D]
+        L.orc_morse_forces.restype = _D
+        L.orc_spline_build.argtypes = [C.POINTER(OrcSpline)]
+        L.orc_spline_eval.argtypes = [C.POINTER(OrcSpline), _D]
+        L.orc_spline_eval.restype = _D
+        L.orc_spline_eval_deriv.argtypes = [C.POINTER(OrcSpline), _D]
+        L.orc_spline_eval_deriv.restype = _D
+        L.orc_eam_read_setfl.argtypes = [C.POINTER(OrcEam), C.c_char_p, C.c_char_p, _I]
+        L.orc_eam_from_arrays.argtypes = [C.POINTER(OrcEam), _I, _I, _D, _I, _D, _D, _PD, _PD,
+                                          _PD]
+        L.orc_eam_forces.argtypes = [C.POINTER(OrcEam), _PD, _PI, _PD, _I64, C.POINTER(OrcBox),
+                                     C.POINTER(OrcNlist), _PD, _PD, _PD]
+        L.orc_eam_forces.restype = _D
+        L.orc_eam_free.argtypes = [C.POINTER(OrcEam)]
+        L.orc_half_kick.argtypes = [_PD, _PD, _PI, _PD, _I64, _D]
+        L.orc_drift.argtypes = [_PD, _PD, _I64, _D, C.POINTER(OrcBox)]
+        L.orc_drift_single_image.argtypes = [_PD, _PD, _I64, _D, C.POINTER(OrcBox)]
+        L.orc_kinetic_energy.argtypes = [_PD, _PI, _PD, _I64]
+        L.orc_kinetic_energy.restype = _D
+        L.orc_nhc_init.argtypes = [C.POINTER(OrcNhc), _D, _D, _I, _D, _I]
+        L.orc_nhc_half_step.argtypes = [C.POINTER(OrcNhc), _D]
+        L.orc_nhc_half_step.restype = _D
+        L.orc_nhc_energy.argtypes = [C.POINTER(OrcNhc)]
+        L.orc_nhc_energy.restype = _D
+        L.orc_run.argtypes = [C.POINTER(OrcRunCfg), C.POINTER(OrcBox), _PD, _PD, _PD, _PI, _PD,
+                              _I64, C.POINTER(OrcRunOut)]
+        _lib = L
+    return _lib
+
+
+def _pd(a):
+    return a.ctypes.data_as(_PD)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_PI)
+
+
+def box(lo, hi, periodic=(1, 1, 1)) -> OrcBox:
+    b = OrcBox()
+    for a in range(3):
+        b.lo[a], b.hi[a], b.periodic[a] = float(lo[a]), float(hi[a]), int(periodic[a])
+    return b
+
+
+def cells(pos, lo, hi, r_list, periodic=(1, 1, 1)):
+    """CellList::build (src/domain/cell_list.cpp:13-58)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    cl = OrcCells()
+    b = box(lo, hi, periodic)
+    if lib().orc_cells_build(C.byref(cl), _pd(pos), len(pos), C.byref(b), r_list) != 0:
+        raise RuntimeError("orc_cells_build failed")
+    nc = cl.ncells
+    out = dict(n=(cl.ncx, cl.ncy, cl.ncz), cs=tuple(cl.cs),
+               counts=np.ctypeslib.as_array(cl.counts, (nc,)).copy(),
+               offsets=np.ctypeslib.as_array(cl.offsets, (nc,)).copy(),
+               atoms=np.ctypeslib.as_array(cl.atoms, (max(len(pos), 1),))[:len(pos)].copy())
+    lib().orc_cells_free(C.byref(cl))
+    return out
+
+
+class HalfList:
+    """NeighborList (src/neighbors/neighbor_list.cpp:14-100), owned C state."""
+
+    def __init__(self, pos, lo, hi, r_cut, r_skin, periodic=(1, 1, 1)):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.box = box(lo, hi, periodic)
+        self.nl = OrcNlist()
+        if lib().orc_nlist_build(C.byref(self.nl), _pd(self.pos), len(self.pos),
+                                 C.byref(self.box), r_cut, r_skin) != 0:
+            raise RuntimeError("orc_nlist_build failed")
+        n = len(self.pos)
+        self.counts = np.ctypeslib.as_array(self.nl.counts, (n,)).copy()
+        self.offsets = np.ctypeslib.as_array(self.nl.offsets, (n,)).copy()
+        self.nbrs = np.ctypeslib.as_array(self.nl.nbrs, (max(self.nl.total, 1),))[
+            :self.nl.total].copy()
+
+    def needs_rebuild(self, pos) -> bool:
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        return bool(lib().orc_nlist_needs_rebuild(C.byref(self.nl), _pd(pos), len(pos)))
+
+    def full(self):
+        """Symmetric closure with sorted rows: (counts, offsets, nbrs)."""
+        n = len(self.pos)
+        counts = np.zeros(n, dtype=np.int32)
+        offsets = np.zeros(n, dtype=np.int32)
+        nbrs = np.zeros(max(2 * self.nl.total, 1), dtype=np.int32)
+        lib().orc_nlist_full(C.byref(self.nl), _pi(counts), _pi(offsets), _pi(nbrs))
+        return counts, offsets, nbrs[:2 * self.nl.total]
+
+    def __del__(self):
+        try:
+            lib().orc_nlist_free(C.byref(self.nl))
+        except Exception:
+            pass
+
+
+def bruteforce_full(pos, lo, hi, r_list, periodic=(1, 1, 1)):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = len(pos)
+    counts = np.zeros(n, dtype=np.int32)
+    offsets = np.zeros(n, dtype=np.int32)
+    cap = 400 * n
+    nbrs = np.zeros(cap, dtype=np.int32)
+    b = box(lo, hi, periodic)
+    tot = lib().orc_bruteforce_full(_pd(pos), n, C.byref(b), r_list, _pi(counts), _pi(offsets),
+                                    _pi(nbrs), cap)
+    assert tot <= cap
+    return counts, offsets, nbrs[:tot]
+
+
+def morse_pair(D, alpha, r0, rc, r2):
+    e, f = C.c_double(), C.c_double()
+    lib().orc_morse_pair(D, alpha, r0, rc * rc, r2, C.byref(e), C.byref(f))
+    return e.value, f.value
+
+
+def morse_forces(half: HalfList, D, alpha, r0, rc):
+    """compute_pair_forces with MorsePair (force_compute.cpp:10-56)."""
+    n = len(half.pos)
+    f = np.zeros((n, 3))
+    pe = lib().orc_morse_forces(_pd(half.pos), _pd(f), n, C.byref(half.box), C.byref(half.nl),
+                                D, alpha, r0, rc)
+    return pe, f
+
+
+def spline_coeffs(values, dr):
+    """SplineTable::build_splines (eam_alloy.cpp:19-50) -> (a, b, c, d)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    s = OrcSpline()
+    s.n = len(v)
+    s.dr = dr
+    s.rmin = 0.0
+    s.values = _pd(v)
+    lib().orc_spline_build(C.byref(s))
+    m = len(v) - 1
+    out = tuple(np.ctypeslib.as_array(getattr(s, k), (m,)).copy() for k in "abcd")
+    # a/b/c/d were malloc'd by the oracle; values belong to numpy: free only coeffs.
+    libc = C.CDLL(None)
+    for k in "abcd":
+        libc.free(C.cast(getattr(s, k), C.c_void_p))
+    return out
+
+
+class Eam:
+    """EamAlloy (eam_alloy.cpp:90-295) in the oracle."""
+
+    def __init__(self, tables=None, path=None):
+        self.e = OrcEam()
+        if path is not None:
+            err = C.create_string_buffer(256)
+            if lib().orc_eam_read_setfl(C.byref(self.e), str(path).encode(), err, 256) != 0:
+                raise RuntimeError(err.value.decode())
+        else:
+            t = tables
+            F = np.ascontiguousarray(t.F, dtype=np.float64)
+            rho = np.ascontiguousarray(t.rho, dtype=np.float64)
+            rphi = np.ascontiguousarray(t.rphi, dtype=np.float64)
+            if lib().orc_eam_from_arrays(C.byref(self.e), t.ntypes, t.nrho, t.drho, t.nr, t.dr,
+                                         t.cutoff, _pd(F), _pd(rho), _pd(rphi)) != 0:
+                raise RuntimeError("orc_eam_from_arrays failed")
+
+    @property
+    def cutoff(self):
+        return self.e.cutoff
+
+    def spline(self, kind, idx):
+        s = getattr(self.e, kind)[idx]
+        m = s.n - 1
+        return dict(n=s.n, dr=s.dr, **{k: np.ctypeslib.as_array(getattr(s, k), (m,)).copy()
+                                        for k in "abcd"})
+
+    def eval(self, kind, idx, r, deriv=False):
+        s = C.byref(getattr(self.e, kind)[idx])
+        fn = lib().orc_spline_eval_deriv if deriv else lib().orc_spline_eval
+        return fn(s, float(r))
+
+    def forces(self, half: HalfList, types):
+        n = len(half.pos)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        f = np.zeros((n, 3))
+        rho = np.zeros(n)
+        fp = np.zeros(n)
+        emb = C.c_double()
+        pe = lib().orc_eam_forces(C.byref(self.e), _pd(half.pos), _pi(types), _pd(f), n,
+                                  C.byref(half.box), C.byref(half.nl), _pd(rho), _pd(fp),
+                                  C.byref(emb))
+        return dict(pe=pe, forces=f, rho=rho, fp=fp, embed=emb.value)
+
+    def __del__(self):
+        try:
+            lib().orc_eam_free(C.byref(self.e))
+        except Exception:
+            pass
+
+
+def kinetic_energy(vel, types, masses):
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    masses = np.ascontiguousarray(masses, dtype=np.float64)
+    return lib().orc_kinetic_energy(_pd(vel), _pi(types), _pd(masses), len(vel))
+
+
+def half_kick(vel, forces, types, masses, dt):
+    lib().orc_half_kick(_pd(vel), _pd(np.ascontiguousarray(forces, dtype=np.float64)),
+                        _pi(np.ascontiguousarray(types, dtype=np.int32)),
+                        _pd(np.ascontiguousarray(masses, dtype=np.float64)), len(vel), dt)
+
+
+def drift(pos, vel, lo, hi, dt, single_image=False, periodic=(1, 1, 1)):
+    b = box(lo, hi, periodic)
+    fn = lib().orc_drift_single_image if single_image else lib().orc_drift
+    fn(_pd(pos), _pd(np.ascontiguousarray(vel, dtype=np.float64)), len(pos), dt, C.byref(b))
+
+
+class Nhc:
+    """NoseHooverChain (device_nose_hoover.cu:187-289)."""
+
+    def __init__(self, t_target, t_period, M, dt, n_dof):
+        self.c = OrcNhc()
+        lib().orc_nhc_init(C.byref(self.c), t_target, t_period, M, dt, n_dof)
+
+    def half_step(self, ke):
+        return lib().orc_nhc_half_step(C.byref(self.c), ke)
+
+    def energy(self):
+        return lib().orc_nhc_energy(C.byref(self.c))
+
+    def state(self):
+        M = self.c.M
+        return (np.array(self.c.eta[:M]), np.array(self.c.eta_v[:M]), np.array(self.c.eta_m[:M]))
+
+
+def run(sys, *, morse=None, eam: Eam | None = None, dt=0.001, skin=1.0, nsteps=0,
+        thermo_every=100, rebuild_mode=0, rebuild_every=10, nvt=False, t_target=0.0,
+        t_period=0.1, nhc_length=3):
+    """The driver loop (tdmd_main.cpp:211-254), optionally wrapped in NHC.
+
+    Returns (pos, vel, forces, thermo[k, 6], rebuilds); `sys` is not modified.
+    """
+    pos = np.ascontiguousarray(sys.pos, dtype=np.float64).copy()
+    vel = np.ascontiguousarray(sys.vel, dtype=np.float64).copy()
+    f = np.zeros_like(pos)
+    types = np.ascontiguousarray(sys.types, dtype=np.int32)
+    masses = np.ascontiguousarray(sys.masses, dtype=np.float64)
+    cfg = OrcRunCfg()
+    cfg.use_eam = 1 if eam is not None else 0
+    if morse is not None:
+        for k in range(4):
+            cfg.morse[k] = float(morse[k])
+    if eam is not None:
+        cfg.eam = C.pointer(eam.e)
+    cfg.dt, cfg.skin, cfg.nsteps, cfg.thermo_every = dt, skin, nsteps, thermo_every
+    cfg.rebuild_mode, cfg.rebuild_every = rebuild_mode, rebuild_every
+    cfg.nvt, cfg.t_target, cfg.t_period, cfg.nhc_length = int(nvt), t_target, t_period, nhc_length
+    cap = nsteps // max(thermo_every, 1) + 2
+    th = np.zeros((cap, 6))
+    out = OrcRunOut()
+    out.thermo = _pd(th)
+    out.thermo_cap = cap
+    b = box(sys.lo, sys.hi, sys.periodic)
+    rc = lib().orc_run(C.byref(cfg), C.byref(b), _pd(pos), _pd(vel), _pd(f), _pi(types),
+                       _pd(masses), len(pos), C.byref(out))
+    if rc != 0:
+        raise RuntimeError("orc_run failed")
+    return pos, vel, f, th[:out.nthermo].copy(), out.rebuilds

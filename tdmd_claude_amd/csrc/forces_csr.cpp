@@ -1,0 +1,300 @@
+// forces_csr.cpp -- Morse and EAM forces over a full CSR list: the
+// compute_morse_gpu / DeviceEam::compute boundary
+// (src/potentials/device_morse.cu:32-170, src/potentials/device_eam.cu:72-421).
+//
+// One thread per atom. Per pair: FP64 difference of FP64 positions, then a
+// cast to the force type R (the relative-coordinate trick, device_morse.cu:
+// 69-74), minimum image and the pair math in R. Per-atom sums (force,
+// density, energy) are carried in FP64 whatever R is. Forces are stored,
+// not accumulated (device_morse.cu:110-117); energies are added to
+// *d_energy: one FP64 atomic per block when no workspace is given,
+// otherwise per-block partials summed in a fixed order.
+#include "spline_dev.hpp"
+
+namespace tdmd_hip {
+
+constexpr int kMaxSplines = 64;  // 2*ntypes + ntypes(ntypes+1)/2 for ntypes <= 8
+
+namespace {
+
+template <class R>
+__device__ __forceinline__ void rel_coords(const double* __restrict__ pos, double xi, double yi,
+                                           double zi, int j, const DevBox& box, R* dx, R* dy,
+                                           R* dz) {
+  const R bsx = static_cast<R>(box.size[0]), bsy = static_cast<R>(box.size[1]),
+          bsz = static_cast<R>(box.size[2]);
+  *dx = min_image(static_cast<R>(xi - pos[3 * j]), bsx, static_cast<R>(0.5) * bsx,
+                  box.periodic[0]);
+  *dy = min_image(static_cast<R>(yi - pos[3 * j + 1]), bsy, static_cast<R>(0.5) * bsy,
+                  box.periodic[1]);
+  *dz = min_image(static_cast<R>(zi - pos[3 * j + 2]), bsz, static_cast<R>(0.5) * bsz,
+                  box.periodic[2]);
+}
+
+__device__ __forceinline__ void emit_energy(double e, double* __restrict__ partials,
+                                            double* __restrict__ d_energy) {
+  const double t = block_sum<kBlock>(e);
+  if (threadIdx.x == 0) {
+    if (partials)
+      partials[blockIdx.x] = t;
+    else
+      atomicAdd(d_energy, t);
+  }
+}
+
+template <class R, bool kEnergy>
+__global__ void __launch_bounds__(kBlock)
+morse_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
+                 const int32_t* __restrict__ nbrs, const int32_t* __restrict__ offsets,
+                 const int32_t* __restrict__ counts, int natoms, DevBox box, R D, R alpha, R r0,
+                 R rc_sq, double* __restrict__ partials, double* __restrict__ d_energy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
+  if (i < natoms) {
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int off = offsets[i], cnt = counts[i];
+    for (int k = 0; k < cnt; ++k) {
+      const int j = nbrs[off + k];
+      R dx, dy, dz;
+      rel_coords<R>(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
+      const R r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < rc_sq) {
+        // morse.hpp:37-55 / device_morse.cu:91-106
+        const R r = sqrt(r2);
+        const R e = exp(-alpha * (r - r0));
+        const R ome = static_cast<R>(1) - e;
+        if (kEnergy) pe += static_cast<double>(static_cast<R>(0.5) * D * (ome * ome - static_cast<R>(1)));
+        const R fpair = -(static_cast<R>(2) * D * alpha * ome * e) / r;
+        fx += static_cast<double>(fpair * dx);
+        fy += static_cast<double>(fpair * dy);
+        fz += static_cast<double>(fpair * dz);
+      }
+    }
+    forces[3 * i] = static_cast<R>(fx);
+    forces[3 * i + 1] = static_cast<R>(fy);
+    forces[3 * i + 2] = static_cast<R>(fz);
+  }
+  if (kEnergy) emit_energy(pe, partials, d_energy);
+}
+
+template <class R>
+__device__ __forceinline__ void load_meta(SplineMeta<R>* smeta, const SplineMeta<R>* meta,
+                                          int ns) {
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) smeta[k] = meta[k];
+  __syncthreads();
+}
+
+// Pass 1: rho_i = sum_j rho_{t_j}(r_ij), FP64 sum (device_eam.cu:72-133).
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+eam_density_csr_kernel(const double* __restrict__ pos, const int32_t* __restrict__ types,
+                       const int32_t* __restrict__ nbrs, const int32_t* __restrict__ offsets,
+                       const int32_t* __restrict__ counts, int natoms, int ntypes, DevBox box,
+                       R rc_sq, const SplineMeta<R>* __restrict__ meta,
+                       const typename Quad<R>::type* __restrict__ coeff,
+                       double* __restrict__ rho) {
+  __shared__ SplineMeta<R> smeta[kMaxSplines];
+  load_meta(smeta, meta, 2 * ntypes);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  const int off = offsets[i], cnt = counts[i];
+  double rho_i = 0.0;
+  for (int k = 0; k < cnt; ++k) {
+    const int j = nbrs[off + k];
+    R dx, dy, dz;
+    rel_coords<R>(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
+    const R r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 >= rc_sq) continue;
+    const R r = sqrt(r2);
+    const int tj = types[j] - 1;
+    rho_i += static_cast<double>(spline_value<R>(smeta[ntypes + tj], coeff, r));
+  }
+  rho[i] = rho_i;
+}
+
+// Pass 2: fp_i = F'_{t_i}(rho_i); energy F(rho_i) (device_eam.cu:137-162).
+template <class R, bool kEnergy>
+__global__ void __launch_bounds__(kBlock)
+eam_embed_csr_kernel(const int32_t* __restrict__ types, int natoms, int ntypes,
+                     const SplineMeta<R>* __restrict__ meta,
+                     const typename Quad<R>::type* __restrict__ coeff,
+                     const double* __restrict__ rho, R* __restrict__ fp,
+                     double* __restrict__ partials, double* __restrict__ d_energy) {
+  __shared__ SplineMeta<R> smeta[kMaxSplines];
+  load_meta(smeta, meta, ntypes);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (i < natoms) {
+    const int ti = types[i] - 1;
+    const R rho_i = static_cast<R>(rho[i]);
+    const SplineVal<R> F = spline_both<R>(smeta[ti], coeff, rho_i);
+    fp[i] = F.df;
+    e = static_cast<double>(F.f);
+  }
+  if (kEnergy) emit_energy(e, partials, d_energy);
+}
+
+// Pass 3: F_i = sum_j -(phi' + fp_i rho'_{t_j} + fp_j rho'_{t_i})/r * d
+// (device_eam.cu:176-278), plus 0.5 phi per pair of the full list.
+template <class R, bool kEnergy>
+__global__ void __launch_bounds__(kBlock)
+eam_force_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
+                     const int32_t* __restrict__ types, const int32_t* __restrict__ nbrs,
+                     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
+                     int natoms, int ntypes, DevBox box, R rc_sq,
+                     const SplineMeta<R>* __restrict__ meta,
+                     const typename Quad<R>::type* __restrict__ coeff,
+                     const R* __restrict__ fp, double* __restrict__ partials,
+                     double* __restrict__ d_energy) {
+  __shared__ SplineMeta<R> smeta[kMaxSplines];
+  const int np = ntypes * (ntypes + 1) / 2;
+  load_meta(smeta, meta, 2 * ntypes + np);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
+  if (i < natoms) {
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int ti = types[i] - 1;
+    const R fpi = fp[i];
+    const int off = offsets[i], cnt = counts[i];
+    for (int k = 0; k < cnt; ++k) {
+      const int j = nbrs[off + k];
+      R dx, dy, dz;
+      rel_coords<R>(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
+      const R r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 >= rc_sq) continue;
+      const R r = sqrt(r2);
+      const int tj = types[j] - 1;
+      const int pidx = phi_index(ti, tj, ntypes);
+      R dphi;
+      if (kEnergy) {
+        const SplineVal<R> ph = spline_both<R>(smeta[2 * ntypes + pidx], coeff, r);
+        pe += 0.5 * static_cast<double>(ph.f);
+        dphi = ph.df;
+      } else {
+        dphi = spline_deriv<R>(smeta[2 * ntypes + pidx], coeff, r);
+      }
+      const R drho_j = spline_deriv<R>(smeta[ntypes + tj], coeff, r);
+      const R drho_i = spline_deriv<R>(smeta[ntypes + ti], coeff, r);
+      const R fpair = -(dphi + fpi * drho_j + fp[j] * drho_i) / r;
+      fx += static_cast<double>(fpair * dx);
+      fy += static_cast<double>(fpair * dy);
+      fz += static_cast<double>(fpair * dz);
+    }
+    forces[3 * i] = static_cast<R>(fx);
+    forces[3 * i + 1] = static_cast<R>(fy);
+    forces[3 * i + 2] = static_cast<R>(fz);
+  }
+  if (kEnergy) emit_energy(pe, partials, d_energy);
+}
+
+template <class R>
+int morse_launch(const double* pos, void* forces, const int32_t* nbrs, const int32_t* offsets,
+                 const int32_t* counts, int natoms, const DevBox& box,
+                 const tdmd_morse_params& p, double* d_energy, void* work, hipStream_t stream) {
+  const int grid = grid_for(natoms);
+  double* partials = static_cast<double*>(work);
+  if (d_energy) {
+    morse_csr_kernel<R, true><<<grid, kBlock, 0, stream>>>(
+        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, static_cast<R>(p.d),
+        static_cast<R>(p.alpha), static_cast<R>(p.r0), static_cast<R>(p.rc_sq), partials,
+        d_energy);
+    TDMD_LAUNCH_CHECK();
+    if (partials) return launch_sum_partials(partials, grid, d_energy, 1, stream);
+  } else {
+    morse_csr_kernel<R, false><<<grid, kBlock, 0, stream>>>(
+        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, static_cast<R>(p.d),
+        static_cast<R>(p.alpha), static_cast<R>(p.r0), static_cast<R>(p.rc_sq), nullptr,
+        nullptr);
+    TDMD_LAUNCH_CHECK();
+  }
+  return TDMD_OK;
+}
+
+template <class R>
+int eam_launch(const tdmd_eam_tables* t, const double* pos, void* forces, const int32_t* types,
+               const int32_t* nbrs, const int32_t* offsets, const int32_t* counts, int natoms,
+               const DevBox& box, double* rho, void* fpv, double* d_energy, void* work,
+               hipStream_t stream) {
+  using Q = typename Quad<R>::type;
+  const int grid = grid_for(natoms);
+  const R rc_sq = static_cast<R>(t->cutoff * t->cutoff);
+  const auto* meta = static_cast<const SplineMeta<R>*>(t->d_meta);
+  const auto* coeff = static_cast<const Q*>(t->d_coeff);
+  R* fp = static_cast<R*>(fpv);
+  double* partials = static_cast<double*>(work);
+  eam_density_csr_kernel<R><<<grid, kBlock, 0, stream>>>(pos, types, nbrs, offsets, counts,
+                                                         natoms, t->ntypes, box, rc_sq, meta,
+                                                         coeff, rho);
+  TDMD_LAUNCH_CHECK();
+  if (d_energy) {
+    eam_embed_csr_kernel<R, true><<<grid, kBlock, 0, stream>>>(types, natoms, t->ntypes, meta,
+                                                               coeff, rho, fp, partials, d_energy);
+    TDMD_LAUNCH_CHECK();
+    if (partials) {
+      int rc = launch_sum_partials(partials, grid, d_energy, 1, stream);
+      if (rc) return rc;
+    }
+    eam_force_csr_kernel<R, true><<<grid, kBlock, 0, stream>>>(
+        pos, static_cast<R*>(forces), types, nbrs, offsets, counts, natoms, t->ntypes, box,
+        rc_sq, meta, coeff, fp, partials, d_energy);
+    TDMD_LAUNCH_CHECK();
+    if (partials) return launch_sum_partials(partials, grid, d_energy, 1, stream);
+  } else {
+    eam_embed_csr_kernel<R, false><<<grid, kBlock, 0, stream>>>(types, natoms, t->ntypes, meta,
+                                                                coeff, rho, fp, nullptr, nullptr);
+    TDMD_LAUNCH_CHECK();
+    eam_force_csr_kernel<R, false><<<grid, kBlock, 0, stream>>>(
+        pos, static_cast<R*>(forces), types, nbrs, offsets, counts, natoms, t->ntypes, box,
+        rc_sq, meta, coeff, fp, nullptr, nullptr);
+    TDMD_LAUNCH_CHECK();
+  }
+  return TDMD_OK;
+}
+
+}  // namespace
+
+}  // namespace tdmd_hip
+
+using namespace tdmd_hip;
+
+extern "C" {
+
+int tdmd_morse_force(int32_t precision, const double* d_pos, void* d_forces,
+                     const int32_t* d_neighbors, const int32_t* d_offsets,
+                     const int32_t* d_counts, int32_t natoms, const tdmd_box* box,
+                     const tdmd_morse_params* params, double* d_energy, void* d_work,
+                     void* stream) {
+  if (!box || !params || natoms < 0) return fail(TDMD_E_INVALID, "morse_force: bad arguments");
+  if (natoms == 0) return TDMD_OK;  // device_morse.cu:141
+  if (!d_pos || !d_forces || !d_offsets || !d_counts)
+    return fail(TDMD_E_INVALID, "morse_force: null pointer");
+  const DevBox db = make_devbox(*box);
+  if (precision == TDMD_MIXED)
+    return morse_launch<float>(d_pos, d_forces, d_neighbors, d_offsets, d_counts, natoms, db,
+                               *params, d_energy, d_work, as_stream(stream));
+  if (precision == TDMD_FP64)
+    return morse_launch<double>(d_pos, d_forces, d_neighbors, d_offsets, d_counts, natoms, db,
+                                *params, d_energy, d_work, as_stream(stream));
+  return fail(TDMD_E_INVALID, "morse_force: bad precision");
+}
+
+int tdmd_eam_compute(const tdmd_eam_tables* t, const double* d_pos, void* d_forces,
+                     const int32_t* d_types, const int32_t* d_neighbors,
+                     const int32_t* d_offsets, const int32_t* d_counts, int32_t natoms,
+                     const tdmd_box* box, double* d_rho, void* d_fp, double* d_energy,
+                     void* d_work, void* stream) {
+  if (!t || !box || natoms < 0) return fail(TDMD_E_INVALID, "eam_compute: bad arguments");
+  if (natoms == 0) return TDMD_OK;  // device_eam.cu:345
+  if (!d_pos || !d_forces || !d_types || !d_offsets || !d_counts || !d_rho || !d_fp)
+    return fail(TDMD_E_INVALID, "eam_compute: null pointer");
+  if (t->ntypes > 8) return fail(TDMD_E_INVALID, "eam_compute: at most 8 element types");
+  const DevBox db = make_devbox(*box);
+  if (t->precision == TDMD_MIXED)
+    return eam_launch<float>(t, d_pos, d_forces, d_types, d_neighbors, d_offsets, d_counts,
+                             natoms, db, d_rho, d_fp, d_energy, d_work, as_stream(stream));
+  return eam_launch<double>(t, d_pos, d_forces, d_types, d_neighbors, d_offsets, d_counts,
+                            natoms, db, d_rho, d_fp, d_energy, d_work, as_stream(stream));
+}
+
+}  // extern "C"

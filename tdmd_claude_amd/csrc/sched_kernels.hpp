@@ -1,0 +1,164 @@
+// sched_kernels.hpp -- device side of the FastPipelineScheduler fast path.
+//
+// Layout (all per-atom arrays are in the scheduler's spatial order, which is
+// re-derived at every neighbour-list rebuild):
+//   px,py,pz / vx,vy,vz   FP64 master state (SoA, coalesced)
+//   bx,by,bz              positions at the last rebuild (skin test)
+//   fx,fy,fz              forces in the force type R (kept for the window
+//                         boundary kick and for download)
+//   M  (mirror)           gather record {x, y, z, type} written after every
+//                         drift; read by the density pass (and by Morse)
+//   M2 (mirror)           {x, y, z, fp} written by the density pass; read
+//                         by the EAM force pass -- one load per neighbour
+//                         brings the neighbour's position and F'(rho_j)
+//   nbr                   neighbour indices, wave-interleaved ("ELL"): the
+//                         k-th neighbour of the 64 atoms of one wavefront
+//                         is one contiguous 256-byte line, so every list
+//                         read of a wave is a single coalesced load
+//
+// Mirror encodings:
+//   MirrorF64    double4; pair difference in FP64, minimum image in FP64,
+//                then one cast to R (the reference's relative-coordinate
+//                trick, done before the minimum image so pairs that cross a
+//                periodic face keep full precision)
+//   MirrorFixed  int4 of 32-bit fixed-point coordinates, unit 2^-s A with
+//                L 2^s < 2^31; the pair difference is exact integer
+//                arithmetic (minimum image in integers), then one
+//                int->float conversion. 16 bytes per gather instead of 32.
+#pragma once
+
+#include "spline_dev.hpp"
+
+namespace tdmd_hip {
+namespace sched {
+
+constexpr int kMaxTypes = 8;
+constexpr int kMaxSplines = 64;
+
+struct MirrorParams {
+  double lo[3];
+  double q;       // 2^s (fixed) ; unused for F64
+  float inv_q;    // 2^-s
+  int Lq[3];      // round(L * 2^s)
+  int hq[3];      // Lq / 2
+  double L[3];
+  double hL[3];
+  int periodic[3];
+};
+
+struct MirrorF64 {
+  using T = double4;
+  __device__ __forceinline__ static T encode(const MirrorParams&, double x, double y, double z,
+                                             double w) {
+    return make_double4(x, y, z, w);
+  }
+  __device__ __forceinline__ static T with_w(T m, double w) {
+    m.w = w;
+    return m;
+  }
+  template <class R>
+  __device__ __forceinline__ static void delta(const T& a, const T& b, const MirrorParams& mp,
+                                               R* dx, R* dy, R* dz) {
+    *dx = static_cast<R>(min_image(a.x - b.x, mp.L[0], mp.hL[0], mp.periodic[0]));
+    *dy = static_cast<R>(min_image(a.y - b.y, mp.L[1], mp.hL[1], mp.periodic[1]));
+    *dz = static_cast<R>(min_image(a.z - b.z, mp.L[2], mp.hL[2], mp.periodic[2]));
+  }
+  __device__ __forceinline__ static int type_of(const T& m) { return static_cast<int>(m.w); }
+  template <class R>
+  __device__ __forceinline__ static R w_real(const T& m) {
+    return static_cast<R>(m.w);
+  }
+  template <class R>
+  __device__ __forceinline__ static double pack_real(R v) {
+    return static_cast<double>(v);
+  }
+};
+
+struct MirrorFixed {
+  using T = int4;
+  __device__ __forceinline__ static T encode(const MirrorParams& mp, double x, double y,
+                                             double z, int w) {
+    T m;
+    m.x = static_cast<int>(__double2ll_rn((x - mp.lo[0]) * mp.q));
+    m.y = static_cast<int>(__double2ll_rn((y - mp.lo[1]) * mp.q));
+    m.z = static_cast<int>(__double2ll_rn((z - mp.lo[2]) * mp.q));
+    m.w = w;
+    return m;
+  }
+  __device__ __forceinline__ static T with_w(T m, int w) {
+    m.w = w;
+    return m;
+  }
+  __device__ __forceinline__ static int imin_image(int d, int Lq, int hq, int periodic) {
+    if (periodic) {
+      if (d > hq)
+        d -= Lq;
+      else if (d < -hq)
+        d += Lq;
+    }
+    return d;
+  }
+  template <class R>
+  __device__ __forceinline__ static void delta(const T& a, const T& b, const MirrorParams& mp,
+                                               R* dx, R* dy, R* dz) {
+    const R s = static_cast<R>(mp.inv_q);
+    *dx = static_cast<R>(imin_image(a.x - b.x, mp.Lq[0], mp.hq[0], mp.periodic[0])) * s;
+    *dy = static_cast<R>(imin_image(a.y - b.y, mp.Lq[1], mp.hq[1], mp.periodic[1])) * s;
+    *dz = static_cast<R>(imin_image(a.z - b.z, mp.Lq[2], mp.hq[2], mp.periodic[2])) * s;
+  }
+  __device__ __forceinline__ static int type_of(const T& m) { return m.w; }
+  template <class R>
+  __device__ __forceinline__ static R w_real(const T& m) {
+    return __int_as_float(m.w);
+  }
+  template <class R>
+  __device__ __forceinline__ static int pack_real(R v) {
+    return __float_as_int(static_cast<float>(v));
+  }
+};
+
+// ELL index of the k-th neighbour of atom i.
+__device__ __forceinline__ size_t ell(int i, int k, int cap) {
+  return (static_cast<size_t>(i >> 6) * cap + k) * 64 + (i & 63);
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+// run of atoms (spatially sorted) so neighbour gathers stay in its L2.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+  const int b = blockIdx.x;
+  const int per = nblocks >> 3;  // nblocks is a multiple of 8
+  return (b & 7) * per + (b >> 3);
+}
+
+// Epilogue of the force pass.
+enum Epilogue : int {
+  kForceOnly = 0,    // store F (initial forces, thermo)
+  kFullKickDrift = 1,// v += dt F/(m mvv2e); x += dt v (cross-step fused kick)
+  kHalfKick = 2,     // v += dt/2 F/(m mvv2e); store F (window end)
+  kKick2Ke = 3       // NVT: v += dt/2 F/m; store F, v; KE partials
+};
+
+struct StepArgs {
+  int natoms;
+  int nblocks;  // grid size (multiple of 8)
+  int cap;
+  int ntypes;
+  double dt;
+  double skin_half_sq;
+  MirrorParams mp;
+  // state
+  double *px, *py, *pz, *vx, *vy, *vz, *bx, *by, *bz;
+  void *fx, *fy, *fz;
+  const int32_t* types;
+  const double* masses;  // by 1-based type
+  const int32_t* nbr;
+  const int32_t* cnt;
+  void* M;
+  void* M2;
+  int32_t* flags;        // [0] rebuild, [1] overflow max, [2] nonfinite
+  double* partials;      // per-block FP64 partial sums
+  double* nhc;           // device NHC state block
+};
+
+}  // namespace sched
+}  // namespace tdmd_hip

@@ -1,0 +1,601 @@
+// sched_kernels.inl -- kernels of the FastPipelineScheduler fast path
+// (src/scheduler/fast_pipeline_scheduler.cu:87-232 re-designed for gfx950).
+//
+// Per NVE step (the reference's kick-drift / force pair,
+// fast_pipeline_scheduler.cu:101-166):
+//   [rebuild: bin, scan, scatter, order, permute, copy-back, list]
+//   EAM density pass        M  -> M2 (rho, F'(rho) fused: reference passes
+//                                 1+2, device_eam.cu:72-162)
+//   force pass + epilogue   M2 -> F, then the cross-step kick and drift of
+//                                 the next step in registers (the
+//                                 reference's fused_full_kick_drift,
+//                                 device_velocity_verlet.cu:88-130), which
+//                                 writes the next M and tests the skin.
+// The density pass reads M and writes M2; the force pass reads M2 and writes
+// M, so no pass reads a record another thread of the same pass writes.
+#pragma once
+
+#include "nhc.hpp"
+#include "sched_kernels.hpp"
+
+namespace tdmd_hip {
+namespace sched {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wrap1(double x, double lo, double L, int periodic) {
+  // device_velocity_verlet.cu:49-61
+  if (periodic) {
+    if (x < lo)
+      x += L;
+    else if (x >= lo + L)
+      x -= L;
+  }
+  return x;
+}
+
+template <class Mir>
+__device__ __forceinline__ typename Mir::T encode_type(const MirrorParams& mp, double x,
+                                                       double y, double z, int ti) {
+  return Mir::encode(mp, x, y, z, ti);
+}
+
+template <class R>
+__device__ __forceinline__ R* rp(void* p) {
+  return static_cast<R*>(p);
+}
+
+__device__ __forceinline__ void emit_partial(double v, double* partials, int slot) {
+  const double t = block_sum<kBlock>(v);
+  if (threadIdx.x == 0) partials[slot] = t;
+}
+
+// Kick-and-drift for atom i with a given kick factor and an optional pre
+// scale of v (NHC). Writes x, v and the M record; sets the rebuild flag when
+// the atom has moved more than skin/2 since the last build (minimum image).
+template <class R, class Mir>
+__device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vscale,
+                                           double kick_dt, double f0, double f1, double f2) {
+  const int ti = a.types[i] - 1;
+  const double mass = a.masses[ti + 1];
+  const double factor = kick_dt / (mass * kMvv2e);
+  const double vx = vscale * a.vx[i] + factor * f0;
+  const double vy = vscale * a.vy[i] + factor * f1;
+  const double vz = vscale * a.vz[i] + factor * f2;
+  const double x = wrap1(a.px[i] + a.dt * vx, a.mp.lo[0], a.mp.L[0], a.mp.periodic[0]);
+  const double y = wrap1(a.py[i] + a.dt * vy, a.mp.lo[1], a.mp.L[1], a.mp.periodic[1]);
+  const double z = wrap1(a.pz[i] + a.dt * vz, a.mp.lo[2], a.mp.L[2], a.mp.periodic[2]);
+  a.vx[i] = vx;
+  a.vy[i] = vy;
+  a.vz[i] = vz;
+  a.px[i] = x;
+  a.py[i] = y;
+  a.pz[i] = z;
+  static_cast<typename Mir::T*>(a.M)[i] = encode_type<Mir>(a.mp, x, y, z, ti);
+  const double dx = min_image(x - a.bx[i], a.mp.L[0], a.mp.hL[0], a.mp.periodic[0]);
+  const double dy = min_image(y - a.by[i], a.mp.L[1], a.mp.hL[1], a.mp.periodic[1]);
+  const double dz = min_image(z - a.bz[i], a.mp.L[2], a.mp.hL[2], a.mp.periodic[2]);
+  if (dx * dx + dy * dy + dz * dz > a.skin_half_sq) a.flags[0] = 1;
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) a.flags[2] = 1;
+}
+
+// Force-pass epilogue.
+template <class R, class Mir, int kEpi>
+__device__ __forceinline__ double epilogue(const StepArgs& a, int i, double fx, double fy,
+                                           double fz) {
+  const R f0 = static_cast<R>(fx), f1 = static_cast<R>(fy), f2 = static_cast<R>(fz);
+  if (kEpi == kFullKickDrift) {
+    kick_drift<R, Mir>(a, i, 1.0, a.dt, static_cast<double>(f0), static_cast<double>(f1),
+                       static_cast<double>(f2));
+    return 0.0;
+  }
+  rp<R>(a.fx)[i] = f0;
+  rp<R>(a.fy)[i] = f1;
+  rp<R>(a.fz)[i] = f2;
+  if (kEpi == kForceOnly) return 0.0;
+  // kHalfKick / kKick2Ke: v += (dt/2) F / (m mvv2e)  (half_kick_kernel,
+  // device_velocity_verlet.cu:14-32)
+  const int ti = a.types[i] - 1;
+  const double mass = a.masses[ti + 1];
+  const double factor = (0.5 * a.dt) / (mass * kMvv2e);
+  const double vx = a.vx[i] + factor * static_cast<double>(f0);
+  const double vy = a.vy[i] + factor * static_cast<double>(f1);
+  const double vz = a.vz[i] + factor * static_cast<double>(f2);
+  a.vx[i] = vx;
+  a.vy[i] = vy;
+  a.vz[i] = vz;
+  if (kEpi == kKick2Ke) return 0.5 * mass * kMvv2e * (vx * vx + vy * vy + vz * vz);
+  return 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// integrate-only kernels
+// ---------------------------------------------------------------------------
+
+// Window bootstrap (fused_half_kick_drift) and the NVT first kick: kick by
+// dt/2 with the stored forces, drift, write M. NVT scales v by the chain's
+// pending factor first.
+template <class R, class Mir, bool kNvt, bool kFull>
+__global__ void __launch_bounds__(kBlock) k_kick_drift(StepArgs a) {
+  const int i = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  const double s = kNvt ? a.nhc[26] : 1.0;
+  kick_drift<R, Mir>(a, i, s, kFull ? a.dt : 0.5 * a.dt, static_cast<double>(rp<R>(a.fx)[i]),
+                     static_cast<double>(rp<R>(a.fy)[i]), static_cast<double>(rp<R>(a.fz)[i]));
+}
+
+// Mirror refresh (after upload or restore): M[i] = {x, type}.
+template <class Mir>
+__global__ void __launch_bounds__(kBlock) k_write_mirror(StepArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  static_cast<typename Mir::T*>(a.M)[i] =
+      encode_type<Mir>(a.mp, a.px[i], a.py[i], a.pz[i], a.types[i] - 1);
+}
+
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_ke_partials(StepArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double e = 0.0;
+  if (i < a.natoms) {
+    const double mass = a.masses[a.types[i]];
+    e = 0.5 * mass * kMvv2e * (a.vx[i] * a.vx[i] + a.vy[i] * a.vy[i] + a.vz[i] * a.vz[i]);
+  }
+  emit_partial(e, a.partials, blockIdx.x);
+}
+
+__global__ void k_scale_v(StepArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  const double s = a.nhc[24];
+  a.vx[i] *= s;
+  a.vy[i] *= s;
+  a.vz[i] *= s;
+}
+
+// ---------------------------------------------------------------------------
+// NHC chain (one thread; device_nose_hoover.cu:206-289 on device state)
+//   nhc[0..7] eta, [8..15] eta_v, [16..23] eta_m, [24] vscale (pending
+//   factor not yet applied to v), [25] KE of the stored v, [26] kick scale,
+//   [27] reduced sum scratch
+// ---------------------------------------------------------------------------
+
+struct NhcParams {
+  int M;
+  int n_dof;
+  double t_target, t_period, dt;
+};
+
+// Before the VV step: half-step on the KE of the actual velocities
+// (vscale^2 * stored KE), kick scale = s1 * vscale.
+__global__ void k_nhc_pre(double* nhc, NhcParams p) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double ke = nhc[24] * nhc[24] * nhc[25];
+  const double s1 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
+                                  p.n_dof, ke);
+  nhc[26] = s1 * nhc[24];
+  nhc[24] = 1.0;
+}
+
+// After the VV step: KE from the per-block partials of the kick-2 epilogue,
+// half-step, pending scale = s2.
+__global__ void k_nhc_post(double* nhc, const double* partials, int n, NhcParams p) {
+  __shared__ double s[256];
+  double t = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) t += partials[k];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double ke = s[0];
+    const double s2 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
+                                    p.n_dof, ke);
+    nhc[25] = ke;
+    nhc[24] = s2;
+  }
+}
+
+// Stored-KE refresh (after upload, or after the pending scale was applied).
+__global__ void k_nhc_set_ke(double* nhc, const double* partials, int n, int commit_scale) {
+  __shared__ double s[256];
+  double t = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) t += partials[k];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    nhc[25] = s[0];
+    if (commit_scale) nhc[24] = 1.0;
+  }
+}
+
+// Window end: the pending factor has been applied to v by k_scale_v.
+__global__ void k_nhc_commit(double* nhc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  nhc[25] *= nhc[24] * nhc[24];
+  nhc[24] = 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// EAM passes
+// ---------------------------------------------------------------------------
+
+template <class R>
+__device__ __forceinline__ void load_meta(SplineMeta<R>* sm, const SplineMeta<R>* meta, int ns) {
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) sm[k] = meta[k];
+  __syncthreads();
+}
+
+template <class R, class Mir, bool kEnergy>
+__global__ void __launch_bounds__(kBlock)
+k_eam_density(StepArgs a, const SplineMeta<R>* __restrict__ meta,
+              const typename Quad<R>::type* __restrict__ coeff, R rc_sq) {
+  using T = typename Mir::T;
+  __shared__ SplineMeta<R> sm[kMaxSplines];
+  load_meta(sm, meta, 2 * a.ntypes);
+  const int blk = xcd_block(a.nblocks);
+  const int i = blk * kBlock + threadIdx.x;
+  const T* __restrict__ M = static_cast<const T*>(a.M);
+  double e = 0.0;
+  if (i < a.natoms) {
+    const T mi = M[i];
+    const int ti = Mir::type_of(mi);
+    const int n = a.cnt[i];
+    double rho = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const int j = a.nbr[ell(i, k, a.cap)];
+      const T mj = M[j];
+      R dx, dy, dz;
+      Mir::template delta<R>(mi, mj, a.mp, &dx, &dy, &dz);
+      const R r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < rc_sq) {
+        const R r = sqrt(r2);
+        rho += static_cast<double>(spline_value<R>(sm[a.ntypes + Mir::type_of(mj)], coeff, r));
+      }
+    }
+    const SplineVal<R> F = spline_both<R>(sm[ti], coeff, static_cast<R>(rho));
+    static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
+    e = static_cast<double>(F.f);
+  }
+  if (kEnergy) emit_partial(e, a.partials, blk);
+}
+
+template <class R, class Mir, int kEpi, bool kEnergy, bool kMulti>
+__global__ void __launch_bounds__(kBlock)
+k_eam_force(StepArgs a, const SplineMeta<R>* __restrict__ meta,
+            const typename Quad<R>::type* __restrict__ coeff, R rc_sq) {
+  using T = typename Mir::T;
+  __shared__ SplineMeta<R> sm[kMaxSplines];
+  const int nt = a.ntypes;
+  load_meta(sm, meta, 2 * nt + nt * (nt + 1) / 2);
+  const int blk = xcd_block(a.nblocks);
+  const int i = blk * kBlock + threadIdx.x;
+  const T* __restrict__ M2 = static_cast<const T*>(a.M2);
+  double red = 0.0;
+  if (i < a.natoms) {
+    const T mi = M2[i];
+    const R fpi = Mir::template w_real<R>(mi);
+    const int ti = kMulti ? a.types[i] - 1 : 0;
+    const int n = a.cnt[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const int j = a.nbr[ell(i, k, a.cap)];
+      const T mj = M2[j];
+      R dx, dy, dz;
+      Mir::template delta<R>(mi, mj, a.mp, &dx, &dy, &dz);
+      const R r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < rc_sq) {
+        const R r = sqrt(r2);
+        const int tj = kMulti ? a.types[j] - 1 : 0;
+        const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
+        R dphi;
+        if (kEnergy) {
+          const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, r);
+          pe += 0.5 * static_cast<double>(ph.f);
+          dphi = ph.df;
+        } else {
+          dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, r);
+        }
+        const R drho_j = spline_deriv<R>(sm[nt + tj], coeff, r);
+        const R drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, r) : drho_j;
+        const R fpj = Mir::template w_real<R>(mj);
+        const R fpair = -(dphi + fpi * drho_j + fpj * drho_i) / r;
+        fx += static_cast<double>(fpair * dx);
+        fy += static_cast<double>(fpair * dy);
+        fz += static_cast<double>(fpair * dz);
+      }
+    }
+    const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
+    red = kEnergy ? pe : ke;
+  }
+  if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
+}
+
+// ---------------------------------------------------------------------------
+// Morse pass (morse.hpp:37-55 pair math in R, FP64 per-atom sums)
+// ---------------------------------------------------------------------------
+
+struct MorseR {
+  double D, alpha, r0, rc_sq;
+};
+
+template <class R, class Mir, int kEpi, bool kEnergy>
+__global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseR p) {
+  // The Morse pass gathers from M, which the kick-drift epilogue rewrites.
+  static_assert(kEpi != kFullKickDrift, "Morse kick-drift runs as its own launch");
+  using T = typename Mir::T;
+  const int blk = xcd_block(a.nblocks);
+  const int i = blk * kBlock + threadIdx.x;
+  const T* __restrict__ M = static_cast<const T*>(a.M);
+  const R D = static_cast<R>(p.D), alpha = static_cast<R>(p.alpha), r0 = static_cast<R>(p.r0),
+          rc_sq = static_cast<R>(p.rc_sq);
+  double red = 0.0;
+  if (i < a.natoms) {
+    const T mi = M[i];
+    const int n = a.cnt[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const int j = a.nbr[ell(i, k, a.cap)];
+      const T mj = M[j];
+      R dx, dy, dz;
+      Mir::template delta<R>(mi, mj, a.mp, &dx, &dy, &dz);
+      const R r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < rc_sq) {
+        const R r = sqrt(r2);
+        const R e = exp(-alpha * (r - r0));
+        const R ome = static_cast<R>(1) - e;
+        if (kEnergy)
+          pe += static_cast<double>(static_cast<R>(0.5) * D * (ome * ome - static_cast<R>(1)));
+        const R fpair = -(static_cast<R>(2) * D * alpha * ome * e) / r;
+        fx += static_cast<double>(fpair * dx);
+        fy += static_cast<double>(fpair * dy);
+        fz += static_cast<double>(fpair * dz);
+      }
+    }
+    const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
+    red = kEnergy ? pe : ke;
+  }
+  if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
+}
+
+// ---------------------------------------------------------------------------
+// Rebuild: spatial sort and the ELL neighbour list. Every kernel first reads
+// the rebuild flag and returns when it is clear, so the whole sequence can
+// sit in a captured graph and only does work on steps that need it.
+// ---------------------------------------------------------------------------
+
+struct GridArgs {
+  int n[3];
+  double inv_cs[3];
+  int ncells;
+  const int32_t* rank_of;  // linear cell id -> Morton rank
+  int32_t* ccount;         // [ncells] by rank
+  int32_t* cstart;         // [ncells] by rank
+  int32_t* cursor;         // [ncells] by rank
+  int32_t* key;            // [N] rank of each atom's cell (old order)
+  int32_t* skey;           // [N] rank of each atom's cell (new order)
+  int32_t* sidx;           // [N] new slot -> old index
+  int32_t* orig;           // [N] upload index
+  int32_t* done;           // completion counter
+};
+
+
+struct PermArgs {
+  // scratch arrays the permutation gathers into
+  double *px, *py, *pz, *vx, *vy, *vz;
+  void *fx, *fy, *fz;
+  int32_t *types, *orig;
+};
+
+__global__ void k_set_flag(int32_t* flags) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) flags[0] = 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero_cells(const int32_t* flags, GridArgs g) {
+  if (!flags[0]) return;
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= g.ncells) return;
+  g.ccount[r] = 0;
+  g.cursor[r] = 0;
+}
+
+__device__ __forceinline__ void cell_coords(const StepArgs& a, const GridArgs& g, double x,
+                                            double y, double z, int c[3]) {
+  const double p[3] = {x, y, z};
+  for (int d = 0; d < 3; ++d) {
+    int v = static_cast<int>((p[d] - a.mp.lo[d]) * g.inv_cs[d]);
+    v = v >= g.n[d] ? g.n[d] - 1 : v;
+    c[d] = v < 0 ? 0 : v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin(StepArgs a, GridArgs g) {
+  if (!a.flags[0]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  int c[3];
+  cell_coords(a, g, a.px[i], a.py[i], a.pz[i], c);
+  const int rank = g.rank_of[(c[2] * g.n[1] + c[1]) * g.n[0] + c[0]];
+  g.key[i] = rank;
+  atomicAdd(&g.ccount[rank], 1);
+}
+
+// Single-block exclusive scan of the cell counts (atomics-free).
+__global__ void __launch_bounds__(1024) k_cell_scan(const int32_t* flags, GridArgs g) {
+  if (!flags[0]) return;
+  __shared__ int wsum[16];
+  __shared__ int carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int start = 0; start < g.ncells; start += 1024) {
+    const int c = start + static_cast<int>(threadIdx.x);
+    const int v = c < g.ncells ? g.ccount[c] : 0;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int base = carry_s;
+    for (int k = 0; k < w; ++k) base += wsum[k];
+    if (c < g.ncells) g.cstart[c] = base + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int k = 0; k < 16; ++k) t += wsum[k];
+      carry_s += t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(StepArgs a, GridArgs g) {
+  if (!a.flags[0]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  const int r = g.key[i];
+  g.sidx[g.cstart[r] + atomicAdd(&g.cursor[r], 1)] = i;
+}
+
+// Deterministic order inside a cell: by upload index.
+__global__ void __launch_bounds__(kBlock) k_order(StepArgs a, GridArgs g) {
+  if (!a.flags[0]) return;
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= g.ncells) return;
+  int32_t* seg = g.sidx + g.cstart[r];
+  const int n = g.ccount[r];
+  for (int x = 1; x < n; ++x) {
+    const int32_t v = seg[x];
+    const int32_t kv = g.orig[v];
+    int y = x - 1;
+    while (y >= 0 && g.orig[seg[y]] > kv) {
+      seg[y + 1] = seg[y];
+      --y;
+    }
+    seg[y + 1] = v;
+  }
+}
+
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_permute(StepArgs a, GridArgs g, PermArgs t) {
+  if (!a.flags[0]) return;
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= a.natoms) return;
+  const int o = g.sidx[s];
+  t.px[s] = a.px[o];
+  t.py[s] = a.py[o];
+  t.pz[s] = a.pz[o];
+  t.vx[s] = a.vx[o];
+  t.vy[s] = a.vy[o];
+  t.vz[s] = a.vz[o];
+  rp<R>(t.fx)[s] = rp<R>(a.fx)[o];
+  rp<R>(t.fy)[s] = rp<R>(a.fy)[o];
+  rp<R>(t.fz)[s] = rp<R>(a.fz)[o];
+  t.types[s] = a.types[o];
+  t.orig[s] = g.orig[o];
+  g.skey[s] = g.key[o];
+}
+
+template <class R, class Mir>
+__global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, PermArgs t) {
+  if (!a.flags[0]) return;
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= a.natoms) return;
+  const double x = t.px[s], y = t.py[s], z = t.pz[s];
+  a.px[s] = x;
+  a.py[s] = y;
+  a.pz[s] = z;
+  a.bx[s] = x;
+  a.by[s] = y;
+  a.bz[s] = z;
+  a.vx[s] = t.vx[s];
+  a.vy[s] = t.vy[s];
+  a.vz[s] = t.vz[s];
+  rp<R>(a.fx)[s] = rp<R>(t.fx)[s];
+  rp<R>(a.fy)[s] = rp<R>(t.fy)[s];
+  rp<R>(a.fz)[s] = rp<R>(t.fz)[s];
+  const int ty = t.types[s];
+  const_cast<int32_t*>(a.types)[s] = ty;
+  g.orig[s] = t.orig[s];
+  static_cast<typename Mir::T*>(a.M)[s] = Mir::encode(a.mp, x, y, z, ty - 1);
+}
+
+// ELL list over the sorted atoms. Inclusion exactly as the CPU list
+// (neighbor_list.cpp:58-62): d = x_j - x_i in FP64, single minimum image,
+// unfused r2 < r_list^2. kCount: only count (capacity sizing).
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock)
+k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, int32_t* nbr, int32_t* cnt,
+            int32_t* maxcnt) {
+  if (!kCount && !a.flags[0]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  int count = 0;
+  if (i < a.natoms) {
+    const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
+    int c[3];
+    cell_coords(a, g, xi, yi, zi, c);
+    for (int dz = -1; dz <= 1; ++dz) {
+      int nz = c[2] + dz;
+      nz = nz < 0 ? nz + g.n[2] : (nz >= g.n[2] ? nz - g.n[2] : nz);
+      for (int dy = -1; dy <= 1; ++dy) {
+        int ny = c[1] + dy;
+        ny = ny < 0 ? ny + g.n[1] : (ny >= g.n[1] ? ny - g.n[1] : ny);
+        for (int dx = -1; dx <= 1; ++dx) {
+          int nx = c[0] + dx;
+          nx = nx < 0 ? nx + g.n[0] : (nx >= g.n[0] ? nx - g.n[0] : nx);
+          const int r = g.rank_of[(nz * g.n[1] + ny) * g.n[0] + nx];
+          const int start = g.cstart[r], end = start + g.ccount[r];
+          for (int j = start; j < end; ++j) {
+            if (j == i) continue;
+            const double ddx = min_image(a.px[j] - xi, a.mp.L[0], a.mp.hL[0], a.mp.periodic[0]);
+            const double ddy = min_image(a.py[j] - yi, a.mp.L[1], a.mp.hL[1], a.mp.periodic[1]);
+            const double ddz = min_image(a.pz[j] - zi, a.mp.L[2], a.mp.hL[2], a.mp.periodic[2]);
+            if (r2_unfused(ddx, ddy, ddz) < r_list_sq) {
+              if (!kCount && count < a.cap) nbr[ell(i, count, a.cap)] = j;
+              ++count;
+            }
+          }
+        }
+      }
+    }
+    if (!kCount) cnt[i] = count < a.cap ? count : a.cap;
+  }
+  const int m = wave_max_i(count);
+  if ((threadIdx.x & 63) == 0 && m > 0) {
+    if (kCount) {
+      atomicMax(maxcnt, m);
+    } else if (m > a.cap) {
+      atomicMax(&a.flags[1], m);
+    }
+  }
+}
+
+// End of a rebuild: the last block to finish clears the flag (every block has
+// read it by then), so the next step's kernels see a clean state.
+__global__ void __launch_bounds__(kBlock) k_rebuild_done(int32_t* flags, int32_t* done,
+                                                         int nblocks, int64_t* rebuilds) {
+  if (!flags[0]) return;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int prev = atomicAdd(done, 1);
+    if (prev == nblocks - 1) {
+      *done = 0;
+      flags[0] = 0;
+      *rebuilds += 1;
+    }
+  }
+}
+
+}  // namespace sched
+}  // namespace tdmd_hip

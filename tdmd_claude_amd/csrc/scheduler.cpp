@@ -1,0 +1,815 @@
+// scheduler.cpp -- FastPipelineScheduler on gfx950
+// (src/scheduler/fast_pipeline_scheduler.cuh:36-192,
+//  src/scheduler/fast_pipeline_scheduler.cu:15-232).
+//
+// Same contract as the reference: construct with a box, an atom count and a
+// potential; upload; run_until(step) (velocity Verlet with the cross-step
+// fused kick, one stream, one synchronisation per call); download in upload
+// order. Re-designed underneath:
+//   * atoms are kept in a Morton-ordered cell sort, re-derived at each
+//     neighbour rebuild on the device (no host round trip, no
+//     cudaDeviceSynchronize -- compare device_cell_list.cu:103-111,148);
+//   * neighbour lists are wave-interleaved (ELL) for coalesced reads;
+//   * EAM runs as two passes per step: density+embedding, then force with
+//     the next step's kick and drift fused into its epilogue;
+//   * the rebuild trigger is decided on the device (fixed interval as the
+//     reference, or max displacement > skin/2), so whole step sequences are
+//     captured once as hipGraphs and replayed;
+//   * NVT (Nose-Hoover chain, device_nose_hoover.cu:187-289) runs on the
+//     device with the KE reduced from the kick epilogue's block partials.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "sched_kernels.inl"
+
+namespace tdmd_hip {
+
+int read_setfl(const std::string& path, SetflData* out);
+
+namespace {
+
+using namespace sched;
+
+// Operation codes of a step sequence (also the graph-cache key alphabet).
+enum Op : char {
+  OP_BOOT = 'b',        // half kick + drift (window start, NVE)
+  OP_KD_FULL = 'k',     // full kick + drift (Morse NVE, steps 2..n)
+  OP_NVT_PRE = 'p',     // chain half-step before the step
+  OP_KD_NVT = 'q',      // scaled half kick + drift
+  OP_REB_FORCED = 'R',  // set flag + rebuild
+  OP_REB_COND = 'r',    // rebuild if the device flag is set
+  OP_DENS = 'd',        // EAM density pass
+  OP_F_FULL = 'F',      // force + next step's kick-drift (EAM)
+  OP_F_ONLY = 'o',      // force, store F
+  OP_F_HALF = 'h',      // force + closing half kick
+  OP_F_KICK2 = '2',     // force + half kick + KE partials (NVT)
+  OP_NVT_POST = 'P',    // chain half-step after the step
+  OP_NVT_FINAL = 'Z'    // apply pending chain scale, commit
+};
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+uint32_t spread3(uint32_t v) {
+  uint32_t r = 0;
+  for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Engine: precision- and mirror-specific launches.
+// ---------------------------------------------------------------------------
+struct EngineBase {
+  virtual ~EngineBase() = default;
+  virtual int launch(char op, hipStream_t s) = 0;
+  virtual int write_mirror(hipStream_t s) = 0;
+  virtual int energy_pass(hipStream_t s, double* d_out) = 0;  // d_out[0..2]
+  virtual int count_pass(hipStream_t s, int32_t* d_max) = 0;
+  virtual int fill_pass(hipStream_t s) = 0;
+  virtual int sort_pass(hipStream_t s) = 0;
+};
+
+}  // namespace tdmd_hip
+
+struct tdmd_sched {
+  int kind = 0;  // 0 Morse, 1 EAM
+  tdmd_box box{};
+  int natoms = 0;
+  int nblocks = 0;
+  tdmd_sched_config cfg{};
+  tdmd_morse_params morse{};
+  tdmd_eam_tables* tables = nullptr;
+  hipStream_t stream = nullptr;
+  double cutoff = 0, r_list = 0;
+  tdmd_hip::sched::StepArgs a{};
+  tdmd_hip::sched::GridArgs g{};
+  tdmd_hip::sched::PermArgs t{};
+  tdmd_hip::sched::NhcParams nhcp{};
+  std::unique_ptr<tdmd_hip::EngineBase> eng;
+  std::vector<void*> allocs;
+  int32_t* d_maxcnt = nullptr;
+  int64_t* d_rebuilds = nullptr;
+  double* d_scalars = nullptr;
+  std::vector<int32_t> h_types, h_ids;
+  std::vector<double> h_masses;
+  bool uploaded = false, initial_forces = false;
+  int current_step = 0, steps_since = 0;
+  int64_t ticks = 0, launches = 0, host_rebuilds = 0, graph_launches = 0;
+  std::map<std::string, hipGraphExec_t> graphs;
+  std::map<std::string, hipGraph_t> graph_defs;
+  std::map<std::string, int64_t> graph_kernels;
+};
+
+namespace tdmd_hip {
+namespace {
+
+template <class R, class Mir>
+struct Engine final : EngineBase {
+  tdmd_sched* S;
+  explicit Engine(tdmd_sched* s) : S(s) {}
+
+  using Q = typename Quad<R>::type;
+  const SplineMeta<R>* meta() const {
+    return static_cast<const SplineMeta<R>*>(S->tables->d_meta);
+  }
+  const Q* coeff() const { return static_cast<const Q*>(S->tables->d_coeff); }
+  R rc_sq() const { return static_cast<R>(S->tables->cutoff * S->tables->cutoff); }
+  MorseR mp() const {
+    return MorseR{S->morse.d, S->morse.alpha, S->morse.r0, S->morse.rc_sq};
+  }
+
+  template <int kEpi, bool kEnergy>
+  void force(hipStream_t s) {
+    const int nb = S->nblocks;
+    if (S->kind == 1) {
+      if (S->tables->ntypes > 1)
+        k_eam_force<R, Mir, kEpi, kEnergy, true><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+      else
+        k_eam_force<R, Mir, kEpi, kEnergy, false><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+    } else {
+      if constexpr (kEpi != kFullKickDrift)
+        k_morse_force<R, Mir, kEpi, kEnergy><<<nb, kBlock, 0, s>>>(S->a, mp());
+    }
+  }
+
+  int sort_pass(hipStream_t s) override {
+    const int gb = grid_for(S->natoms), gc = grid_for(S->g.ncells);
+    k_zero_cells<<<gc, kBlock, 0, s>>>(S->a.flags, S->g);
+    k_bin<<<gb, kBlock, 0, s>>>(S->a, S->g);
+    k_cell_scan<<<1, 1024, 0, s>>>(S->a.flags, S->g);
+    k_scatter<<<gb, kBlock, 0, s>>>(S->a, S->g);
+    k_order<<<gc, kBlock, 0, s>>>(S->a, S->g);
+    k_permute<R><<<gb, kBlock, 0, s>>>(S->a, S->g, S->t);
+    k_copyback<R, Mir><<<gb, kBlock, 0, s>>>(S->a, S->g, S->t);
+    S->launches += 7;
+    TDMD_LAUNCH_CHECK();
+    return TDMD_OK;
+  }
+  int fill_pass(hipStream_t s) override {
+    const int gb = grid_for(S->natoms);
+    k_nlist_ell<false><<<gb, kBlock, 0, s>>>(S->a, S->g, S->r_list * S->r_list,
+                                             const_cast<int32_t*>(S->a.nbr),
+                                             const_cast<int32_t*>(S->a.cnt), nullptr);
+    k_rebuild_done<<<gb, kBlock, 0, s>>>(S->a.flags, S->g.done, gb, S->d_rebuilds);
+    S->launches += 2;
+    TDMD_LAUNCH_CHECK();
+    return TDMD_OK;
+  }
+  int count_pass(hipStream_t s, int32_t* d_max) override {
+    k_nlist_ell<true><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a, S->g, S->r_list * S->r_list,
+                                                             nullptr, nullptr, d_max);
+    TDMD_LAUNCH_CHECK();
+    return TDMD_OK;
+  }
+  int write_mirror(hipStream_t s) override {
+    k_write_mirror<Mir><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a);
+    TDMD_LAUNCH_CHECK();
+    return TDMD_OK;
+  }
+  int energy_pass(hipStream_t s, double* d_out) override {
+    const int nb = S->nblocks;
+    if (S->kind == 1) {
+      k_eam_density<R, Mir, true><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+      launch_sum_partials(S->a.partials, nb, d_out + 0, 0, s);
+    } else {
+      tdmd_fill_u32(d_out, 0u, 2, s);
+    }
+    force<kForceOnly, true>(s);
+    launch_sum_partials(S->a.partials, nb, d_out + 1, 0, s);
+    k_ke_partials<R><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a);
+    launch_sum_partials(S->a.partials, grid_for(S->natoms), d_out + 2, 0, s);
+    TDMD_LAUNCH_CHECK();
+    return TDMD_OK;
+  }
+
+  int launch(char op, hipStream_t s) override {
+    const int nb = S->nblocks;
+    const int gb = grid_for(S->natoms);
+    switch (op) {
+      case OP_BOOT:
+        k_kick_drift<R, Mir, false, false><<<nb, kBlock, 0, s>>>(S->a);
+        break;
+      case OP_KD_FULL:
+        k_kick_drift<R, Mir, false, true><<<nb, kBlock, 0, s>>>(S->a);
+        break;
+      case OP_KD_NVT:
+        k_kick_drift<R, Mir, true, false><<<nb, kBlock, 0, s>>>(S->a);
+        break;
+      case OP_NVT_PRE:
+        k_nhc_pre<<<1, 64, 0, s>>>(S->a.nhc, S->nhcp);
+        break;
+      case OP_NVT_POST:
+        k_nhc_post<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, nb, S->nhcp);
+        break;
+      case OP_NVT_FINAL:
+        k_scale_v<<<gb, kBlock, 0, s>>>(S->a);
+        k_nhc_commit<<<1, 64, 0, s>>>(S->a.nhc);
+        ++S->launches;
+        break;
+      case OP_REB_FORCED: {
+        k_set_flag<<<1, 64, 0, s>>>(S->a.flags);
+        ++S->launches;
+        const int rc = sort_pass(s);
+        return rc ? rc : fill_pass(s);
+      }
+      case OP_REB_COND: {
+        const int rc = sort_pass(s);
+        return rc ? rc : fill_pass(s);
+      }
+      case OP_DENS:
+        k_eam_density<R, Mir, false><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+        break;
+      case OP_F_FULL:
+        force<kFullKickDrift, false>(s);
+        break;
+      case OP_F_ONLY:
+        force<kForceOnly, false>(s);
+        break;
+      case OP_F_HALF:
+        force<kHalfKick, false>(s);
+        break;
+      case OP_F_KICK2:
+        force<kKick2Ke, false>(s);
+        break;
+      default:
+        return fail(TDMD_E_STATE, std::string("scheduler: unknown op ") + op);
+    }
+    ++S->launches;
+    TDMD_LAUNCH_CHECK();
+    return TDMD_OK;
+  }
+};
+
+template <class T>
+int dalloc(tdmd_sched* S, T** p, size_t n) {
+  void* q = nullptr;
+  const int rc = tdmd_malloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (rc) return rc;
+  S->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return TDMD_OK;
+}
+
+int sched_alloc(tdmd_sched* S) {
+  const int N = S->natoms;
+  const size_t fsz = S->cfg.precision == TDMD_MIXED ? sizeof(float) : sizeof(double);
+  StepArgs& a = S->a;
+  int rc = 0;
+  double** d9[9] = {&a.px, &a.py, &a.pz, &a.vx, &a.vy, &a.vz, &a.bx, &a.by, &a.bz};
+  for (auto* p : d9)
+    if ((rc = dalloc(S, p, N))) return rc;
+  void** f3[3] = {&a.fx, &a.fy, &a.fz};
+  for (auto* p : f3) {
+    char* q;
+    if ((rc = dalloc(S, &q, N * fsz))) return rc;
+    *p = q;
+  }
+  int32_t* types;
+  if ((rc = dalloc(S, &types, N))) return rc;
+  a.types = types;
+  double* masses;
+  if ((rc = dalloc(S, &masses, 64))) return rc;
+  a.masses = masses;
+  const bool fixed = S->cfg.precision == TDMD_MIXED && !S->cfg.mirror_fp64;
+  const size_t msz = fixed ? 16 : 32;
+  char *M, *M2;
+  if ((rc = dalloc(S, &M, N * msz))) return rc;
+  if ((rc = dalloc(S, &M2, N * msz))) return rc;
+  a.M = M;
+  a.M2 = M2;
+  if ((rc = dalloc(S, &a.flags, 8))) return rc;
+  if ((rc = dalloc(S, &a.partials, S->nblocks + 64))) return rc;
+  if ((rc = dalloc(S, &a.nhc, 64))) return rc;
+  int32_t* cnt;
+  if ((rc = dalloc(S, &cnt, round_up(N, 64)))) return rc;
+  a.cnt = cnt;
+  // permutation scratch
+  PermArgs& t = S->t;
+  double** t6[6] = {&t.px, &t.py, &t.pz, &t.vx, &t.vy, &t.vz};
+  for (auto* p : t6)
+    if ((rc = dalloc(S, p, N))) return rc;
+  void** tf[3] = {&t.fx, &t.fy, &t.fz};
+  for (auto* p : tf) {
+    char* q;
+    if ((rc = dalloc(S, &q, N * fsz))) return rc;
+    *p = q;
+  }
+  if ((rc = dalloc(S, &t.types, N))) return rc;
+  if ((rc = dalloc(S, &t.orig, N))) return rc;
+  GridArgs& g = S->g;
+  int32_t* rank_of;
+  if ((rc = dalloc(S, &rank_of, g.ncells))) return rc;
+  g.rank_of = rank_of;
+  if ((rc = dalloc(S, &g.ccount, g.ncells))) return rc;
+  if ((rc = dalloc(S, &g.cstart, g.ncells))) return rc;
+  if ((rc = dalloc(S, &g.cursor, g.ncells))) return rc;
+  if ((rc = dalloc(S, &g.key, N))) return rc;
+  if ((rc = dalloc(S, &g.skey, N))) return rc;
+  if ((rc = dalloc(S, &g.sidx, N))) return rc;
+  if ((rc = dalloc(S, &g.orig, N))) return rc;
+  if ((rc = dalloc(S, &g.done, 4))) return rc;
+  if ((rc = dalloc(S, &S->d_maxcnt, 4))) return rc;
+  if ((rc = dalloc(S, &S->d_rebuilds, 1))) return rc;
+  if ((rc = dalloc(S, &S->d_scalars, 16))) return rc;
+  // Morton rank of every cell of the binning grid.
+  std::vector<std::pair<uint32_t, int>> order(static_cast<size_t>(g.ncells));
+  for (int z = 0, c = 0; z < g.n[2]; ++z)
+    for (int y = 0; y < g.n[1]; ++y)
+      for (int x = 0; x < g.n[0]; ++x, ++c)
+        order[static_cast<size_t>(c)] = {spread3(x) | (spread3(y) << 1) | (spread3(z) << 2), c};
+  std::sort(order.begin(), order.end());
+  std::vector<int32_t> h_rank(static_cast<size_t>(g.ncells));
+  for (int r = 0; r < g.ncells; ++r) h_rank[static_cast<size_t>(order[static_cast<size_t>(r)].second)] = r;
+  TDMD_HIP_TRY(hipMemcpy(rank_of, h_rank.data(), h_rank.size() * 4, hipMemcpyHostToDevice));
+  TDMD_HIP_TRY(hipMemset(g.ccount, 0, static_cast<size_t>(g.ncells) * 4));
+  TDMD_HIP_TRY(hipMemset(g.cursor, 0, static_cast<size_t>(g.ncells) * 4));
+  TDMD_HIP_TRY(hipMemset(g.done, 0, 16));
+  TDMD_HIP_TRY(hipMemset(a.flags, 0, 32));
+  TDMD_HIP_TRY(hipMemset(S->d_rebuilds, 0, 8));
+  TDMD_HIP_TRY(hipMemset(a.nhc, 0, 64 * 8));
+  return TDMD_OK;
+}
+
+int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched_config& cfg) {
+  if (natoms <= 0) return fail(TDMD_E_INVALID, "scheduler: natoms must be positive");
+  if (!(cfg.dt > 0) || !(cfg.r_skin >= 0) || cfg.rebuild_every < 1)
+    return fail(TDMD_E_INVALID, "scheduler: bad dt / r_skin / rebuild_every");
+  if (cfg.precision != TDMD_FP64 && cfg.precision != TDMD_MIXED)
+    return fail(TDMD_E_INVALID, "scheduler: bad precision");
+  if (cfg.t_target > 0 && (cfg.nhc_length < 1 || cfg.nhc_length > 8 || !(cfg.t_period > 0)))
+    return fail(TDMD_E_INVALID, "scheduler: NHC chain length must be 1..8, t_period > 0");
+  S->box = box;
+  S->natoms = natoms;
+  S->cfg = cfg;
+  S->nblocks = round_up(grid_for(natoms), 8);
+  S->r_list = S->cutoff + cfg.r_skin;
+  MirrorParams& mp = S->a.mp;
+  double Lmax = 0;
+  for (int d = 0; d < 3; ++d) {
+    mp.lo[d] = box.lo[d];
+    mp.L[d] = box.hi[d] - box.lo[d];
+    mp.hL[d] = 0.5 * mp.L[d];
+    mp.periodic[d] = box.periodic[d] ? 1 : 0;
+    if (!(mp.L[d] > 0)) return fail(TDMD_E_INVALID, "scheduler: box must have positive size");
+    Lmax = std::max(Lmax, mp.L[d]);
+  }
+  // 32-bit fixed point: unit 2^-s with (1.001 Lmax) 2^s < 2^31.
+  int s = static_cast<int>(std::floor(std::log2(2147483647.0 / (Lmax * 1.001))));
+  s = std::min(s, 60);
+  mp.q = std::ldexp(1.0, s);
+  mp.inv_q = static_cast<float>(std::ldexp(1.0, -s));
+  for (int d = 0; d < 3; ++d) {
+    mp.Lq[d] = static_cast<int>(std::llround(mp.L[d] * mp.q));
+    mp.hq[d] = mp.Lq[d] / 2;
+  }
+  // Binning grid: cells no smaller than r_list (with a relative margin so
+  // rounding in the bin computation can never push a neighbour out of the
+  // 27-cell stencil), at least 3 per axis (cell_list.cpp:22-27).
+  GridArgs& g = S->g;
+  for (int d = 0; d < 3; ++d) {
+    g.n[d] = std::max(3, static_cast<int>(std::floor(mp.L[d] / (S->r_list * (1.0 + 1e-9)))));
+    g.inv_cs[d] = static_cast<double>(g.n[d]) / mp.L[d];
+    if (g.n[d] > 1024) return fail(TDMD_E_INVALID, "scheduler: more than 1024 cells per axis");
+  }
+  g.ncells = g.n[0] * g.n[1] * g.n[2];
+  S->a.natoms = natoms;
+  S->a.nblocks = S->nblocks;
+  S->a.dt = cfg.dt;
+  S->a.skin_half_sq = 0.25 * cfg.r_skin * cfg.r_skin;
+  S->nhcp.M = cfg.t_target > 0 ? cfg.nhc_length : 0;
+  S->nhcp.n_dof = 3 * natoms - 3;  // pipeline_scheduler.cu:38
+  S->nhcp.t_target = cfg.t_target;
+  S->nhcp.t_period = cfg.t_period;
+  S->nhcp.dt = cfg.dt;
+  TDMD_HIP_TRY(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  int rc = sched_alloc(S);
+  if (rc) return rc;
+  const bool fixed = cfg.precision == TDMD_MIXED && !cfg.mirror_fp64;
+  if (cfg.precision == TDMD_FP64)
+    S->eng.reset(new Engine<double, MirrorF64>(S));
+  else if (fixed)
+    S->eng.reset(new Engine<float, MirrorFixed>(S));
+  else
+    S->eng.reset(new Engine<float, MirrorF64>(S));
+  return TDMD_OK;
+}
+
+void sched_free(tdmd_sched* S) {
+  if (S->stream) (void)hipStreamSynchronize(S->stream);
+  for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
+  for (void* p : S->allocs) (void)hipFree(p);
+  if (S->a.nbr) (void)hipFree(const_cast<int32_t*>(S->a.nbr));
+  if (S->tables) eam_tables_destroy(S->tables);
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+}
+
+int check_flags(tdmd_sched* S) {
+  int32_t f[4];
+  TDMD_HIP_TRY(hipMemcpy(f, S->a.flags, 16, hipMemcpyDeviceToHost));
+  if (f[2]) return fail(TDMD_E_NONFINITE, "scheduler: non-finite position detected");
+  if (f[1] > S->a.cap)
+    return fail(TDMD_E_OVERFLOW, "scheduler: neighbour capacity " + std::to_string(S->a.cap) +
+                                     " exceeded (max count " + std::to_string(f[1]) + ")");
+  return TDMD_OK;
+}
+
+// First neighbour build: sort, count exactly, size the list, fill.
+int first_build(tdmd_sched* S) {
+  hipStream_t s = S->stream;
+  k_set_flag<<<1, 64, 0, s>>>(S->a.flags);
+  TDMD_LAUNCH_CHECK();
+  int rc = S->eng->sort_pass(s);
+  if (rc) return rc;
+  TDMD_HIP_TRY(hipMemsetAsync(S->d_maxcnt, 0, 4, s));
+  if ((rc = S->eng->count_pass(s, S->d_maxcnt))) return rc;
+  int32_t maxc = 0;
+  TDMD_HIP_TRY(hipMemcpyAsync(&maxc, S->d_maxcnt, 4, hipMemcpyDeviceToHost, s));
+  TDMD_HIP_TRY(hipStreamSynchronize(s));
+  int cap = S->cfg.max_neighbors > 0 ? S->cfg.max_neighbors
+                                     : round_up(static_cast<int>(std::ceil(maxc * 1.25)) + 8, 4);
+  cap = std::max(cap, 4);
+  if (S->a.nbr == nullptr || cap > S->a.cap) {
+    if (S->a.nbr) TDMD_HIP_TRY(hipFree(const_cast<int32_t*>(S->a.nbr)));
+    void* p = nullptr;
+    rc = tdmd_malloc(&p, static_cast<size_t>(round_up(S->natoms, 64)) * cap * 4);
+    if (rc) return rc;
+    S->a.nbr = static_cast<int32_t*>(p);
+    S->a.cap = cap;
+    for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
+    S->graphs.clear();
+    S->graph_defs.clear();
+    S->graph_kernels.clear();
+  }
+  rc = S->eng->fill_pass(s);
+  if (rc) return rc;
+  ++S->host_rebuilds;
+  return TDMD_OK;
+}
+
+// Enqueue a sequence of ops, as one cached graph when enabled.
+int run_ops(tdmd_sched* S, const std::string& ops) {
+  if (ops.empty()) return TDMD_OK;
+  hipStream_t s = S->stream;
+  if (!S->cfg.use_graph) {
+    for (char op : ops) {
+      const int rc = S->eng->launch(op, s);
+      if (rc) return rc;
+    }
+    return TDMD_OK;
+  }
+  auto it = S->graphs.find(ops);
+  if (it == S->graphs.end()) {
+    const int64_t saved = S->launches;
+    TDMD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (char op : ops) {
+      const int rc = S->eng->launch(op, s);
+      if (rc) {
+        hipGraph_t dummy;
+        (void)hipStreamEndCapture(s, &dummy);
+        return rc;
+      }
+    }
+    hipGraph_t gdef;
+    TDMD_HIP_TRY(hipStreamEndCapture(s, &gdef));
+    hipGraphExec_t exec;
+    TDMD_HIP_TRY(hipGraphInstantiate(&exec, gdef, nullptr, nullptr, 0));
+    S->graph_defs[ops] = gdef;
+    S->graph_kernels[ops] = S->launches - saved;
+    it = S->graphs.emplace(ops, exec).first;
+    S->launches = saved;
+  }
+  S->launches += S->graph_kernels[ops];  // kernels the graph replays
+  TDMD_HIP_TRY(hipGraphLaunch(it->second, s));
+  ++S->graph_launches;
+  return TDMD_OK;
+}
+
+int compute_initial_forces(tdmd_sched* S) {
+  // fast_pipeline_scheduler.cu:168-193: build the list, compute F(t=0).
+  int rc = first_build(S);
+  if (rc) return rc;
+  S->steps_since = 0;
+  hipStream_t s = S->stream;
+  if (S->kind == 1) {
+    if ((rc = S->eng->launch(OP_DENS, s))) return rc;
+  }
+  if ((rc = S->eng->launch(OP_F_ONLY, s))) return rc;
+  if (S->nhcp.M > 0) {
+    // Stored KE of the uploaded velocities; no pending chain scale.
+    k_ke_partials<double><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a);
+    k_nhc_set_ke<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, grid_for(S->natoms), 0);
+    TDMD_LAUNCH_CHECK();
+    std::vector<double> init(64, 0.0);
+    init[24] = 1.0;
+    TDMD_HIP_TRY(hipMemcpyAsync(S->a.nhc, init.data(), 24 * 8, hipMemcpyHostToDevice, s));
+    TDMD_HIP_TRY(hipMemcpyAsync(S->a.nhc + 24, init.data() + 24, 8, hipMemcpyHostToDevice, s));
+  }
+  S->initial_forces = true;
+  return TDMD_OK;
+}
+
+constexpr int kChunk = 16;  // steps per captured graph
+
+int run_window(tdmd_sched* S, int n) {
+  // Step ops; the rebuild test happens after the kick-drift of each step
+  // (fast_pipeline_scheduler.cu:101-166 + maybe_rebuild_nlist :225-232).
+  const bool nvt = S->nhcp.M > 0;
+  const bool eam = S->kind == 1;
+  const bool disp = S->cfg.rebuild_policy == TDMD_REBUILD_DISPLACEMENT;
+  std::vector<std::string> steps;
+  steps.reserve(static_cast<size_t>(n));
+  for (int k = 1; k <= n; ++k) {
+    std::string st;
+    if (nvt) {
+      st += static_cast<char>(OP_NVT_PRE);
+      st += static_cast<char>(OP_KD_NVT);
+    } else if (k == 1) {
+      st += static_cast<char>(OP_BOOT);
+    } else if (!eam) {
+      st += static_cast<char>(OP_KD_FULL);
+    }
+    if (disp) {
+      st += static_cast<char>(OP_REB_COND);
+    } else if (S->steps_since >= S->cfg.rebuild_every) {
+      st += static_cast<char>(OP_REB_FORCED);
+      S->steps_since = 0;
+      ++S->host_rebuilds;
+    }
+    if (eam) st += static_cast<char>(OP_DENS);
+    if (nvt) {
+      st += static_cast<char>(OP_F_KICK2);
+      st += static_cast<char>(OP_NVT_POST);
+    } else if (k == n) {
+      st += static_cast<char>(OP_F_HALF);
+    } else {
+      st += static_cast<char>(eam ? OP_F_FULL : OP_F_ONLY);
+    }
+    if (nvt && k == n) st += static_cast<char>(OP_NVT_FINAL);
+    steps.push_back(st);
+    ++S->steps_since;
+    ++S->current_step;
+    ++S->ticks;
+  }
+  for (size_t k = 0; k < steps.size(); k += kChunk) {
+    std::string ops;
+    for (size_t q = k; q < std::min(steps.size(), k + kChunk); ++q) ops += steps[q];
+    const int rc = run_ops(S, ops);
+    if (rc) return rc;
+  }
+  return TDMD_OK;
+}
+
+}  // namespace
+}  // namespace tdmd_hip
+
+using namespace tdmd_hip;
+
+extern "C" {
+
+void tdmd_sched_config_default(tdmd_sched_config* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->dt = 0.001;  // FastPipelineConfig defaults, fast_pipeline_scheduler.cuh:42-46
+  cfg->r_skin = 1.0;
+  cfg->rebuild_every = 10;
+  cfg->precision = TDMD_MIXED;
+  cfg->rebuild_policy = TDMD_REBUILD_FIXED;
+  cfg->t_target = 0.0;
+  cfg->t_period = 0.1;
+  cfg->nhc_length = 3;
+  cfg->use_graph = 1;
+  cfg->mirror_fp64 = 0;
+}
+
+int tdmd_sched_create_morse(tdmd_sched** out, const tdmd_box* box, int32_t natoms,
+                            const tdmd_morse_params* params, const tdmd_sched_config* cfg) {
+  if (!out || !box || !params || !cfg) return fail(TDMD_E_INVALID, "sched_create_morse: null");
+  if (!(params->rc > 0)) return fail(TDMD_E_INVALID, "sched_create_morse: rc must be positive");
+  auto* S = new tdmd_sched();
+  S->kind = 0;
+  S->morse = *params;
+  S->cutoff = params->rc;  // fast_pipeline_scheduler.cu:23 (cutoff_ = params.rc)
+  const int rc = sched_setup(S, *box, natoms, *cfg);
+  if (rc) {
+    sched_free(S);
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return TDMD_OK;
+}
+
+int tdmd_sched_create_eam(tdmd_sched** out, const tdmd_box* box, int32_t natoms, int32_t ntypes,
+                          int32_t nrho, double drho, int32_t nr, double dr, double cutoff,
+                          const double* F, const double* rho, const double* rphi,
+                          const tdmd_sched_config* cfg) {
+  if (!out || !box || !cfg || !F || !rho || !rphi)
+    return fail(TDMD_E_INVALID, "sched_create_eam: null");
+  if (ntypes < 1 || ntypes > sched::kMaxTypes)
+    return fail(TDMD_E_INVALID, "sched_create_eam: 1..8 element types");
+  auto* S = new tdmd_sched();
+  S->kind = 1;
+  int rc = tdmd_eam_tables_create(&S->tables, cfg->precision, ntypes, nrho, drho, nr, dr, cutoff,
+                                  F, rho, rphi);
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  S->cutoff = cutoff;  // fast_pipeline_scheduler.cu:35 (cutoff_ = eam.cutoff())
+  rc = sched_setup(S, *box, natoms, *cfg);
+  if (rc) {
+    sched_free(S);
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return TDMD_OK;
+}
+
+int tdmd_sched_destroy(tdmd_sched* s) {
+  if (!s) return TDMD_OK;
+  sched_free(s);
+  delete s;
+  return TDMD_OK;
+}
+
+int tdmd_sched_upload(tdmd_sched* S, const double* pos, const double* vel, const double* forces,
+                      const int32_t* types, const int32_t* ids, const double* masses,
+                      int32_t n_masses) {
+  if (!S || !pos || !vel || !types || !masses || n_masses < 2 || n_masses > 64)
+    return fail(TDMD_E_INVALID, "sched_upload: bad arguments (masses indexed by 1-based type)");
+  const int N = S->natoms;
+  const int ntypes_max = S->kind == 1 ? S->tables->ntypes : n_masses - 1;
+  for (int i = 0; i < N; ++i)
+    if (types[i] < 1 || types[i] > ntypes_max || types[i] >= n_masses)
+      return fail(TDMD_E_INVALID, "sched_upload: atom type out of range at index " +
+                                      std::to_string(i));
+  for (int k = 1; k < n_masses; ++k)
+    if (!(masses[k] > 0)) return fail(TDMD_E_INVALID, "sched_upload: masses must be positive");
+  S->h_types.assign(types, types + N);
+  S->h_ids.assign(N, 0);
+  if (ids) S->h_ids.assign(ids, ids + N);
+  S->h_masses.assign(masses, masses + n_masses);
+  const bool mixed = S->cfg.precision == TDMD_MIXED;
+  std::vector<double> buf(static_cast<size_t>(N));
+  std::vector<float> fbuf(static_cast<size_t>(N));
+  double* pd[3] = {S->a.px, S->a.py, S->a.pz};
+  double* vd[3] = {S->a.vx, S->a.vy, S->a.vz};
+  void* fd[3] = {S->a.fx, S->a.fy, S->a.fz};
+  for (int d = 0; d < 3; ++d) {
+    for (int i = 0; i < N; ++i) buf[static_cast<size_t>(i)] = pos[3 * i + d];
+    TDMD_HIP_TRY(hipMemcpy(pd[d], buf.data(), N * 8, hipMemcpyHostToDevice));
+    for (int i = 0; i < N; ++i) buf[static_cast<size_t>(i)] = vel[3 * i + d];
+    TDMD_HIP_TRY(hipMemcpy(vd[d], buf.data(), N * 8, hipMemcpyHostToDevice));
+    for (int i = 0; i < N; ++i) buf[static_cast<size_t>(i)] = forces ? forces[3 * i + d] : 0.0;
+    if (mixed) {
+      for (int i = 0; i < N; ++i) fbuf[static_cast<size_t>(i)] = static_cast<float>(buf[static_cast<size_t>(i)]);
+      TDMD_HIP_TRY(hipMemcpy(fd[d], fbuf.data(), N * 4, hipMemcpyHostToDevice));
+    } else {
+      TDMD_HIP_TRY(hipMemcpy(fd[d], buf.data(), N * 8, hipMemcpyHostToDevice));
+    }
+  }
+  TDMD_HIP_TRY(hipMemcpy(const_cast<int32_t*>(S->a.types), types, N * 4, hipMemcpyHostToDevice));
+  std::vector<double> m64(64, 1.0);
+  std::copy(masses, masses + n_masses, m64.begin());
+  TDMD_HIP_TRY(hipMemcpy(const_cast<double*>(S->a.masses), m64.data(), 64 * 8, hipMemcpyHostToDevice));
+  std::vector<int32_t> iota(static_cast<size_t>(N));
+  std::iota(iota.begin(), iota.end(), 0);
+  TDMD_HIP_TRY(hipMemcpy(S->g.orig, iota.data(), N * 4, hipMemcpyHostToDevice));
+  TDMD_HIP_TRY(hipMemset(S->a.flags, 0, 32));
+  const int rc = S->eng->write_mirror(S->stream);
+  if (rc) return rc;
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  // fast_pipeline_scheduler.cu:70-72
+  S->uploaded = true;
+  S->initial_forces = false;
+  S->current_step = 0;
+  S->steps_since = 0;
+  return TDMD_OK;
+}
+
+int tdmd_sched_run_until(tdmd_sched* S, int32_t target_step) {
+  if (!S) return fail(TDMD_E_INVALID, "sched_run_until: null");
+  if (!S->uploaded) return fail(TDMD_E_STATE, "sched_run_until: upload() first");
+  int rc;
+  if (!S->initial_forces && (rc = compute_initial_forces(S))) return rc;
+  if (S->current_step < target_step) {
+    rc = run_window(S, target_step - S->current_step);
+    if (rc) return rc;
+  }
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  return check_flags(S);
+}
+
+int tdmd_sched_download(const tdmd_sched* S, double* pos, double* vel, double* forces,
+                        int32_t* types, int32_t* ids, int32_t natoms) {
+  if (!S || natoms != S->natoms) return fail(TDMD_E_INVALID, "sched_download: natoms mismatch");
+  if (!S->uploaded) return fail(TDMD_E_STATE, "sched_download: nothing uploaded");
+  const int N = S->natoms;
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  std::vector<int32_t> orig(static_cast<size_t>(N));
+  TDMD_HIP_TRY(hipMemcpy(orig.data(), S->g.orig, N * 4, hipMemcpyDeviceToHost));
+  std::vector<double> buf(static_cast<size_t>(N));
+  std::vector<float> fbuf(static_cast<size_t>(N));
+  const double* pd[3] = {S->a.px, S->a.py, S->a.pz};
+  const double* vd[3] = {S->a.vx, S->a.vy, S->a.vz};
+  const void* fd[3] = {S->a.fx, S->a.fy, S->a.fz};
+  const bool mixed = S->cfg.precision == TDMD_MIXED;
+  for (int d = 0; d < 3; ++d) {
+    if (pos) {
+      TDMD_HIP_TRY(hipMemcpy(buf.data(), pd[d], N * 8, hipMemcpyDeviceToHost));
+      for (int s = 0; s < N; ++s) pos[3 * orig[static_cast<size_t>(s)] + d] = buf[static_cast<size_t>(s)];
+    }
+    if (vel) {
+      TDMD_HIP_TRY(hipMemcpy(buf.data(), vd[d], N * 8, hipMemcpyDeviceToHost));
+      for (int s = 0; s < N; ++s) vel[3 * orig[static_cast<size_t>(s)] + d] = buf[static_cast<size_t>(s)];
+    }
+    if (forces) {
+      if (mixed) {
+        TDMD_HIP_TRY(hipMemcpy(fbuf.data(), fd[d], N * 4, hipMemcpyDeviceToHost));
+        for (int s = 0; s < N; ++s)
+          forces[3 * orig[static_cast<size_t>(s)] + d] = fbuf[static_cast<size_t>(s)];
+      } else {
+        TDMD_HIP_TRY(hipMemcpy(buf.data(), fd[d], N * 8, hipMemcpyDeviceToHost));
+        for (int s = 0; s < N; ++s)
+          forces[3 * orig[static_cast<size_t>(s)] + d] = buf[static_cast<size_t>(s)];
+      }
+    }
+  }
+  if (types) std::copy(S->h_types.begin(), S->h_types.end(), types);
+  if (ids) std::copy(S->h_ids.begin(), S->h_ids.end(), ids);
+  return TDMD_OK;
+}
+
+int tdmd_sched_stats(const tdmd_sched* S, int64_t out[4]) {
+  if (!S || !out) return fail(TDMD_E_INVALID, "sched_stats: null");
+  int64_t dev_rebuilds = 0;
+  TDMD_HIP_TRY(hipMemcpy(&dev_rebuilds, S->d_rebuilds, 8, hipMemcpyDeviceToHost));
+  out[0] = S->ticks;
+  out[1] = S->launches;
+  out[2] = dev_rebuilds;
+  out[3] = S->graph_launches;
+  return TDMD_OK;
+}
+
+int tdmd_sched_current_step(const tdmd_sched* S, int32_t* step) {
+  if (!S || !step) return fail(TDMD_E_INVALID, "sched_current_step: null");
+  *step = S->current_step;
+  return TDMD_OK;
+}
+
+int tdmd_sched_interaction_cutoff(const tdmd_sched* S, double* rc) {
+  if (!S || !rc) return fail(TDMD_E_INVALID, "sched_interaction_cutoff: null");
+  *rc = S->cutoff;
+  return TDMD_OK;
+}
+
+int tdmd_sched_stream(const tdmd_sched* S, void** stream) {
+  if (!S || !stream) return fail(TDMD_E_INVALID, "sched_stream: null");
+  *stream = S->stream;
+  return TDMD_OK;
+}
+
+int tdmd_sched_thermo(tdmd_sched* S, double out[6]) {
+  if (!S || !out) return fail(TDMD_E_INVALID, "sched_thermo: null");
+  if (!S->uploaded) return fail(TDMD_E_STATE, "sched_thermo: upload() first");
+  int rc;
+  if (!S->initial_forces && (rc = compute_initial_forces(S))) return rc;
+  if ((rc = S->eng->energy_pass(S->stream, S->d_scalars))) return rc;
+  double sc[3];
+  TDMD_HIP_TRY(hipMemcpyAsync(sc, S->d_scalars, 24, hipMemcpyDeviceToHost, S->stream));
+  std::vector<int32_t> cnt(static_cast<size_t>(S->natoms));
+  TDMD_HIP_TRY(hipMemcpyAsync(cnt.data(), S->a.cnt, S->natoms * 4, hipMemcpyDeviceToHost, S->stream));
+  double nhc[32];
+  TDMD_HIP_TRY(hipMemcpyAsync(nhc, S->a.nhc, 32 * 8, hipMemcpyDeviceToHost, S->stream));
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  out[0] = sc[0] + sc[1];
+  out[1] = sc[2];
+  double enhc = 0.0;
+  const int M = S->nhcp.M;
+  const double kt = kBoltzmann * S->cfg.t_target;
+  for (int k = 0; k < M; ++k)
+    enhc += 0.5 * nhc[16 + k] * nhc[8 + k] * nhc[8 + k] +
+            (k == 0 ? static_cast<double>(S->nhcp.n_dof) * kt : kt) * nhc[k];
+  out[2] = enhc;
+  out[3] = 2.0 * sc[2] / (3.0 * static_cast<double>(S->natoms) * kBoltzmann);
+  int64_t sum = 0;
+  int mx = 0;
+  for (int v : cnt) {
+    sum += v;
+    mx = std::max(mx, v);
+  }
+  out[4] = mx;
+  out[5] = static_cast<double>(sum) / S->natoms;
+  return check_flags(S);
+}
+
+}  // extern "C"
