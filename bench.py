@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""Throughput of the FastPipelineScheduler step on one MI355X.
+
+Workload (the 1M-atom EAM configuration): FCC Cu 64x64x64 cells
+(1,048,576 atoms, a = 3.615 A, box 231.36 A), Johnson-form Cu EAM tabulated
+on 5000-point FD-Hermite splines (cutoff 5.5 A), skin 1.0 A with the
+skin/2 displacement rebuild, 300 K Maxwell-Boltzmann velocities (seed 7),
+NVT Nose-Hoover chain of 3 (tau 0.1 ps, 300 K), dt 1 fs. Synthetic inputs,
+state resident in HBM before the timed region.
+
+One JSON line on stdout: the mixed-precision throughput (value), the FP64
+path's throughput on the same workload, the force error of the mixed path
+against the FP64 CPU oracle, the NVE energy drift, a roofline object for the
+dominant kernel (the EAM force pass) and a CPU baseline (the oracle's FP64
+driver loop on one core, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from tdmd_claude_amd import capi  # noqa: E402
+from tdmd_claude_amd import inputs as I  # noqa: E402
+from tdmd_claude_amd.scheduler import FastPipelineConfig, FastPipelineScheduler  # noqa: E402
+
+METRIC = "Matom-steps/s, 1M-atom EAM Cu, mixed vs FP64 at ≤1e-5 rel force error"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_system(cells: int) -> I.System:
+    s = I.fcc_lattice(cells, cells, cells, 3.615)
+    I.init_velocities(s, 300.0, 7)
+    return s
+
+
+def nvt_cfg(precision: str) -> FastPipelineConfig:
+    return FastPipelineConfig(dt=0.001, r_skin=1.0, precision=precision,
+                              rebuild_policy="displacement", t_target=300.0, t_period=0.1,
+                              nhc_length=3, use_graph=True)
+
+
+def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[float, float]:
+    """Warm up, then time exactly `steps` steps: host clock and HIP events on
+    the scheduler's stream, device synchronised on both sides."""
+    sched.run_until(warmup)
+    stream = sched.stream()
+    e0, e1 = capi.C.c_void_p(), capi.C.c_void_p()
+    capi.call("tdmd_event_create", capi.C.byref(e0))
+    capi.call("tdmd_event_create", capi.C.byref(e1))
+    capi.call("tdmd_device_synchronize")
+    t0 = time.perf_counter()
+    capi.call("tdmd_event_record", e0, stream)
+    sched.run_until(warmup + steps)  # ends with a stream synchronisation
+    capi.call("tdmd_event_record", e1, stream)
+    capi.call("tdmd_device_synchronize")
+    wall = time.perf_counter() - t0
+    ms = capi.C.c_float()
+    capi.call("tdmd_event_elapsed_ms", e0, e1, capi.C.byref(ms))
+    capi.call("tdmd_event_destroy", e0)
+    capi.call("tdmd_event_destroy", e1)
+    return wall, ms.value / 1000.0
+
+
+def bytes_per_atom(precision: str, mirror_fp64: bool, nbar: float) -> dict:
+    """Algorithmic HBM bytes per atom of the two steady-state passes
+    (each per-atom array touched once per pass, neighbour indices read once
+    per pass, neighbour records served from cache)."""
+    rec = 32 if (precision == "fp64" or mirror_fp64) else 16  # M / M2 record
+    density = rec + 4 + 4 * nbar + rec  # read M_i, count, indices; write M2_i
+    # force pass: read M2_i, count, type, x, v, x_build, indices; write x, v, M_i
+    force = rec + 4 + 4 + 24 + 24 + 24 + 4 * nbar + 24 + 24 + rec
+    return dict(density=density, force=force, step=density + force)
+
+
+def nve_drift(system: I.System, tables, precision: str, steps: int, every: int) -> dict:
+    """Mixed NVE run from the given state: slope of total energy per atom."""
+    cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=precision,
+                             rebuild_policy="displacement")
+    sched = FastPipelineScheduler.for_system(system, tables, cfg)
+    t, e = [], []
+    for k in range(0, steps + 1, every):
+        sched.run_until(k)
+        th = sched.thermo()
+        t.append(k * 0.001)
+        e.append((th["pe"] + th["ke"]) / system.natoms)
+    t = np.array(t)
+    e = np.array(e)
+    slope = np.polyfit(t, e, 1)[0]  # eV/atom/ps
+    sched.close()
+    return dict(drift_ev_per_atom_per_ns=float(slope * 1000.0),
+                max_dev_ev_per_atom=float(np.abs(e - e[0]).max()), steps=steps,
+                precision=precision)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cells", type=int, default=64)
+    ap.add_argument("--fp64-steps", type=int, default=500)
+    ap.add_argument("--nve-steps", type=int, default=2000)
+    ap.add_argument("--mirror-fp64", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short mixed run only (for rocprofv3)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the step path runs on a single GPU here; "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if world != 1:
+        print("bench.py is single-process; WORLD_SIZE must be 1", file=sys.stderr)
+        return 2
+
+    capi.call("tdmd_set_device", 0)
+    name = capi.C.create_string_buffer(256)
+    capi.call("tdmd_device_name", name, 256)
+    log("device:", name.value.decode())
+    tables = I.johnson_setfl()
+    system = make_system(args.cells)
+    N = system.natoms
+    log(f"system: {N} atoms, box {system.hi[0]:.2f} A")
+
+    if args.profile:
+        sched = FastPipelineScheduler.for_system(
+            system, tables, FastPipelineConfig(**{**nvt_cfg("mixed").__dict__,
+                                                  "mirror_fp64": args.mirror_fp64}))
+        wall, dev = timed_run(sched, args.warmup, args.steps)
+        log(f"profile run: {args.steps} steps in {wall:.3f} s")
+        nve = FastPipelineScheduler.for_system(
+            system, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
+        ta, tb = nve.time_passes(20)
+        log(f"passes: density {ta:.4f} ms, force {tb:.4f} ms")
+        return 0
+
+    # ---- mixed precision (the metric) ----
+    cfg = nvt_cfg("mixed")
+    cfg.mirror_fp64 = args.mirror_fp64
+    sched = FastPipelineScheduler.for_system(system, tables, cfg)
+    log("mixed: warmup + timed run")
+    wall, dev = timed_run(sched, args.warmup, args.steps)
+    th = sched.thermo()
+    value = N * args.steps / wall / 1e6
+    log(f"mixed: {args.steps} steps, wall {wall:.4f} s (events {dev:.4f} s) -> {value:.1f} "
+        f"Matom-steps/s; T {th['temp']:.1f} K, mean neighbours {th['mean_nbr']:.2f}")
+    stats = sched.stats()
+    pos_m, vel_m, f_m, _, _ = sched.download()
+    sched.close()
+
+    # ---- per-pass timing of the steady-state NVE step (roofline) ----
+    state = system.copy()
+    state.pos, state.vel = pos_m, vel_m
+    nve = FastPipelineScheduler.for_system(
+        state, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
+    nve.run_until(0)
+    nbar = nve.thermo()["mean_nbr"]
+    t_dens, t_force = nve.time_passes(30)
+    nve.close()
+    bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar)
+    achieved = N * bpa["force"] / (t_force * 1e-3) / 1e9
+    log(f"passes: density {t_dens:.4f} ms, force {t_force:.4f} ms; force pass "
+        f"{achieved:.0f} GB/s algorithmic ({bpa['force']:.0f} B/atom)")
+
+    # ---- FP64 path, same workload ----
+    fp64 = None
+    if args.fp64_steps > 0:
+        s64 = FastPipelineScheduler.for_system(system, tables, nvt_cfg("fp64"))
+        log("fp64: warmup + timed run")
+        w64, d64 = timed_run(s64, args.warmup, args.fp64_steps)
+        v64 = N * args.fp64_steps / w64 / 1e6
+        s64.close()
+        fp64 = dict(value=v64, steps=args.fp64_steps, ms_per_step=1000 * w64 / args.fp64_steps)
+        log(f"fp64: {v64:.1f} Matom-steps/s")
+
+    # ---- NVE energy drift (mixed) ----
+    drift = nve_drift(state, tables, "mixed", args.nve_steps, max(args.nve_steps // 10, 1))
+    log(f"NVE drift: {drift['drift_ev_per_atom_per_ns']:.3e} eV/atom/ns")
+
+    # ---- CPU baseline leg: FP64 oracle driver (also the force checker) ----
+    cpu = None
+    force_err = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        eam = O.Eam(tables=tables)
+        ref_state = system.copy()
+        ref_state.pos = pos_m
+        ref_state.vel = vel_m
+        t0 = time.perf_counter()
+        _, _, f_ref, _, _ = O.run(ref_state, eam=eam, nsteps=0, thermo_every=1)
+        t1 = time.perf_counter()
+        rms = float(np.sqrt(np.mean(np.sum(f_ref * f_ref, axis=1))))
+        force_err = float(np.max(np.linalg.norm(f_m - f_ref, axis=1)) / rms)
+        log(f"force error vs FP64 oracle: {force_err:.3e} (max |dF| / RMS |F|, RMS {rms:.3f})")
+        nsteps_cpu = 2
+        t2 = time.perf_counter()
+        O.run(ref_state, eam=eam, nsteps=nsteps_cpu, thermo_every=nsteps_cpu)
+        t3 = time.perf_counter()
+        per_step = ((t3 - t2) - (t1 - t0)) / nsteps_cpu
+        cpu = dict(value=N / per_step / 1e6, unit="Matom-steps/s", cores=1, kind="port",
+                   sample=(f"{N}-atom EAM Cu (the GPU run's state), {nsteps_cpu} steps of the "
+                           "FP64 driver loop (tdmd_main.cpp:237-254) after its list build; "
+                           f"{per_step:.2f} s/step on {os.cpu_count()} host CPUs, 1 used"))
+        log(f"cpu baseline: {cpu['value']:.4f} Matom-steps/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Matom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * wall / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "mixed: f32 tables/pair functions, f64 geometry and sums",
+        "data": "synthetic FCC Cu lattice, seeded 300 K velocities, Johnson-form EAM tables",
+        "config": {
+            "workload": "EAM Cu 64x64x64 FCC cells (1,048,576 atoms), NVT NHC(3) 300 K, "
+                        "dt 1 fs, skin 1.0 A displacement rebuild",
+            "natoms": N,
+            "potential": "Johnson-form Cu EAM, 5000-point FD-Hermite tables, rc 5.5 A",
+            "mirror": "fp64" if args.mirror_fp64 else "fixed32",
+            "mean_neighbours": round(nbar, 2),
+            "rebuilds": stats.rebuilds,
+            "kernel_launches_per_step": round(stats.kernel_launches / max(stats.ticks, 1), 2),
+        },
+        "events_s": round(dev, 5),
+        "fp64": None if fp64 is None else {k: round(v, 4) if isinstance(v, float) else v
+                                           for k, v in fp64.items()},
+        "mixed_over_fp64": None if fp64 is None else round(value / fp64["value"], 3),
+        "force_rel_err_vs_fp64_oracle": force_err,
+        "nve_energy_drift": drift,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_eam_force (force + fused kick-drift)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_atom": round(bpa["force"], 1),
+            "avg_ms": round(t_force, 5),
+            "density_pass_ms": round(t_dens, 5),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

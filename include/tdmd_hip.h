@@ -291,6 +291,12 @@ int tdmd_sched_interaction_cutoff(const tdmd_sched* s, double* rc);
  * out[0] PE, [1] KE, [2] NHC chain energy, [3] T (3N dof), [4] max
  * neighbour count, [5] mean neighbour count. Runs one energy pass. */
 int tdmd_sched_thermo(tdmd_sched* s, double out[6]);
+/* Timing window (no reference counterpart; measurement aid): runs `steps`
+ * NVE steps without rebuild tests, with HIP events around each pass of the
+ * steady-state step, and returns the mean durations: ms_out[0] = EAM
+ * density pass (Morse: kick-drift), ms_out[1] = force pass (EAM: with the
+ * fused kick-drift epilogue). Advances the state like run_until. */
+int tdmd_sched_time_passes(tdmd_sched* s, int32_t steps, float* ms_out);
 /* Stream the scheduler runs on (for event timing). */
 int tdmd_sched_stream(const tdmd_sched* s, void** stream);
 

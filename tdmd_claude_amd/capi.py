@@ -157,6 +157,7 @@ _PROTOS = {
     "tdmd_sched_interaction_cutoff": (C.c_int, [_VP, _PD]),
     "tdmd_sched_thermo": (C.c_int, [_VP, _PD]),
     "tdmd_sched_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "tdmd_sched_time_passes": (C.c_int, [_VP, _I32, C.POINTER(C.c_float)]),
 }
 
 _lib = None

@@ -2,14 +2,13 @@
 // compute_morse_gpu / DeviceEam::compute boundary
 // (src/potentials/device_morse.cu:32-170, src/potentials/device_eam.cu:72-421).
 //
-// One thread per atom. Per pair: FP64 difference of FP64 positions, then a
-// cast to the force type R (the relative-coordinate trick, device_morse.cu:
-// 69-74), minimum image and the pair math in R. Per-atom sums (force,
-// density, energy) are carried in FP64 whatever R is. Forces are stored,
-// not accumulated (device_morse.cu:110-117); energies are added to
-// *d_energy: one FP64 atomic per block when no workspace is given,
-// otherwise per-block partials summed in a fixed order.
-#include "spline_dev.hpp"
+// One thread per atom. Pair geometry (difference, minimum image, r^2, cutoff,
+// r) in FP64, pair functions in R (pair_math.hpp). Per-atom sums (force,
+// density, energy) are FP64. Forces are stored, not accumulated
+// (device_morse.cu:110-117); energies are added to *d_energy: one FP64
+// atomic per block when no workspace is given, otherwise per-block partials
+// summed in a fixed order.
+#include "pair_math.hpp"
 
 namespace tdmd_hip {
 
@@ -17,18 +16,20 @@ constexpr int kMaxSplines = 64;  // 2*ntypes + ntypes(ntypes+1)/2 for ntypes <= 
 
 namespace {
 
-template <class R>
-__device__ __forceinline__ void rel_coords(const double* __restrict__ pos, double xi, double yi,
-                                           double zi, int j, const DevBox& box, R* dx, R* dy,
-                                           R* dz) {
-  const R bsx = static_cast<R>(box.size[0]), bsy = static_cast<R>(box.size[1]),
-          bsz = static_cast<R>(box.size[2]);
-  *dx = min_image(static_cast<R>(xi - pos[3 * j]), bsx, static_cast<R>(0.5) * bsx,
-                  box.periodic[0]);
-  *dy = min_image(static_cast<R>(yi - pos[3 * j + 1]), bsy, static_cast<R>(0.5) * bsy,
-                  box.periodic[1]);
-  *dz = min_image(static_cast<R>(zi - pos[3 * j + 2]), bsz, static_cast<R>(0.5) * bsz,
-                  box.periodic[2]);
+__device__ __forceinline__ double pair_geom(const double* __restrict__ pos, double xi, double yi,
+                                            double zi, int j, const DevBox& box, double* dx,
+                                            double* dy, double* dz) {
+  *dx = min_image(xi - pos[3 * j], box.size[0], 0.5 * box.size[0], box.periodic[0]);
+  *dy = min_image(yi - pos[3 * j + 1], box.size[1], 0.5 * box.size[1], box.periodic[1]);
+  *dz = min_image(zi - pos[3 * j + 2], box.size[2], 0.5 * box.size[2], box.periodic[2]);
+  return *dx * *dx + *dy * *dy + *dz * *dz;
+}
+
+// 0-based table index of a 1-based type. Types outside 1..ntypes (which the
+// reference would use to index out of its tables) are clamped to the range.
+__device__ __forceinline__ int clamp_type(int t, int ntypes) {
+  t -= 1;
+  return t < 0 ? 0 : (t >= ntypes ? ntypes - 1 : t);
 }
 
 __device__ __forceinline__ void emit_energy(double e, double* __restrict__ partials,
@@ -46,28 +47,24 @@ template <class R, bool kEnergy>
 __global__ void __launch_bounds__(kBlock)
 morse_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
                  const int32_t* __restrict__ nbrs, const int32_t* __restrict__ offsets,
-                 const int32_t* __restrict__ counts, int natoms, DevBox box, R D, R alpha, R r0,
-                 R rc_sq, double* __restrict__ partials, double* __restrict__ d_energy) {
+                 const int32_t* __restrict__ counts, int natoms, DevBox box, MorseCoef p,
+                 double* __restrict__ partials, double* __restrict__ d_energy) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
   if (i < natoms) {
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const int off = offsets[i], cnt = counts[i];
     for (int k = 0; k < cnt; ++k) {
-      const int j = nbrs[off + k];
-      R dx, dy, dz;
-      rel_coords<R>(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
-      const R r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < rc_sq) {
-        // morse.hpp:37-55 / device_morse.cu:91-106
-        const R r = sqrt(r2);
-        const R e = exp(-alpha * (r - r0));
-        const R ome = static_cast<R>(1) - e;
-        if (kEnergy) pe += static_cast<double>(static_cast<R>(0.5) * D * (ome * ome - static_cast<R>(1)));
-        const R fpair = -(static_cast<R>(2) * D * alpha * ome * e) / r;
-        fx += static_cast<double>(fpair * dx);
-        fy += static_cast<double>(fpair * dy);
-        fz += static_cast<double>(fpair * dz);
+      double dx, dy, dz;
+      const double r2 = pair_geom(pos, xi, yi, zi, nbrs[off + k], box, &dx, &dy, &dz);
+      if (r2 < p.rc_sq) {
+        const PairR<R> pr = pair_r<R>(r2);
+        R e;
+        const double fpair = static_cast<double>(morse_fpair<R>(p, pr, &e));
+        if (kEnergy) pe += 0.5 * static_cast<double>(e);
+        fx = fma(fpair, dx, fx);
+        fy = fma(fpair, dy, fy);
+        fz = fma(fpair, dz, fz);
       }
     }
     forces[3 * i] = static_cast<R>(fx);
@@ -77,10 +74,8 @@ morse_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
   if (kEnergy) emit_energy(pe, partials, d_energy);
 }
 
-template <class R>
-__device__ __forceinline__ void load_meta(SplineMeta<R>* smeta, const SplineMeta<R>* meta,
-                                          int ns) {
-  for (int k = threadIdx.x; k < ns; k += blockDim.x) smeta[k] = meta[k];
+__device__ __forceinline__ void load_meta(SplineMeta* sm, const SplineMeta* meta, int ns) {
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) sm[k] = meta[k];
   __syncthreads();
 }
 
@@ -90,10 +85,10 @@ __global__ void __launch_bounds__(kBlock)
 eam_density_csr_kernel(const double* __restrict__ pos, const int32_t* __restrict__ types,
                        const int32_t* __restrict__ nbrs, const int32_t* __restrict__ offsets,
                        const int32_t* __restrict__ counts, int natoms, int ntypes, DevBox box,
-                       R rc_sq, const SplineMeta<R>* __restrict__ meta,
+                       double rc_sq, const SplineMeta* __restrict__ meta,
                        const typename Quad<R>::type* __restrict__ coeff,
                        double* __restrict__ rho) {
-  __shared__ SplineMeta<R> smeta[kMaxSplines];
+  __shared__ SplineMeta smeta[kMaxSplines];
   load_meta(smeta, meta, 2 * ntypes);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natoms) return;
@@ -102,13 +97,12 @@ eam_density_csr_kernel(const double* __restrict__ pos, const int32_t* __restrict
   double rho_i = 0.0;
   for (int k = 0; k < cnt; ++k) {
     const int j = nbrs[off + k];
-    R dx, dy, dz;
-    rel_coords<R>(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
-    const R r2 = dx * dx + dy * dy + dz * dz;
+    double dx, dy, dz;
+    const double r2 = pair_geom(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
     if (r2 >= rc_sq) continue;
-    const R r = sqrt(r2);
-    const int tj = types[j] - 1;
-    rho_i += static_cast<double>(spline_value<R>(smeta[ntypes + tj], coeff, r));
+    const PairR<R> pr = pair_r<R>(r2);
+    rho_i += static_cast<double>(
+        spline_value<R>(smeta[ntypes + clamp_type(types[j], ntypes)], coeff, pr.r));
   }
   rho[i] = rho_i;
 }
@@ -117,18 +111,16 @@ eam_density_csr_kernel(const double* __restrict__ pos, const int32_t* __restrict
 template <class R, bool kEnergy>
 __global__ void __launch_bounds__(kBlock)
 eam_embed_csr_kernel(const int32_t* __restrict__ types, int natoms, int ntypes,
-                     const SplineMeta<R>* __restrict__ meta,
+                     const SplineMeta* __restrict__ meta,
                      const typename Quad<R>::type* __restrict__ coeff,
                      const double* __restrict__ rho, R* __restrict__ fp,
                      double* __restrict__ partials, double* __restrict__ d_energy) {
-  __shared__ SplineMeta<R> smeta[kMaxSplines];
+  __shared__ SplineMeta smeta[kMaxSplines];
   load_meta(smeta, meta, ntypes);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0;
   if (i < natoms) {
-    const int ti = types[i] - 1;
-    const R rho_i = static_cast<R>(rho[i]);
-    const SplineVal<R> F = spline_both<R>(smeta[ti], coeff, rho_i);
+    const SplineVal<R> F = spline_both<R>(smeta[clamp_type(types[i], ntypes)], coeff, rho[i]);
     fp[i] = F.df;
     e = static_cast<double>(F.f);
   }
@@ -142,44 +134,44 @@ __global__ void __launch_bounds__(kBlock)
 eam_force_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
                      const int32_t* __restrict__ types, const int32_t* __restrict__ nbrs,
                      const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
-                     int natoms, int ntypes, DevBox box, R rc_sq,
-                     const SplineMeta<R>* __restrict__ meta,
+                     int natoms, int ntypes, DevBox box, double rc_sq,
+                     const SplineMeta* __restrict__ meta,
                      const typename Quad<R>::type* __restrict__ coeff,
                      const R* __restrict__ fp, double* __restrict__ partials,
                      double* __restrict__ d_energy) {
-  __shared__ SplineMeta<R> smeta[kMaxSplines];
+  __shared__ SplineMeta smeta[kMaxSplines];
   const int np = ntypes * (ntypes + 1) / 2;
   load_meta(smeta, meta, 2 * ntypes + np);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
   if (i < natoms) {
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    const int ti = types[i] - 1;
+    const int ti = clamp_type(types[i], ntypes);
     const R fpi = fp[i];
     const int off = offsets[i], cnt = counts[i];
     for (int k = 0; k < cnt; ++k) {
       const int j = nbrs[off + k];
-      R dx, dy, dz;
-      rel_coords<R>(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
-      const R r2 = dx * dx + dy * dy + dz * dz;
+      double dx, dy, dz;
+      const double r2 = pair_geom(pos, xi, yi, zi, j, box, &dx, &dy, &dz);
       if (r2 >= rc_sq) continue;
-      const R r = sqrt(r2);
-      const int tj = types[j] - 1;
+      const PairR<R> pr = pair_r<R>(r2);
+      const int tj = clamp_type(types[j], ntypes);
       const int pidx = phi_index(ti, tj, ntypes);
       R dphi;
       if (kEnergy) {
-        const SplineVal<R> ph = spline_both<R>(smeta[2 * ntypes + pidx], coeff, r);
+        const SplineVal<R> ph = spline_both<R>(smeta[2 * ntypes + pidx], coeff, pr.r);
         pe += 0.5 * static_cast<double>(ph.f);
         dphi = ph.df;
       } else {
-        dphi = spline_deriv<R>(smeta[2 * ntypes + pidx], coeff, r);
+        dphi = spline_deriv<R>(smeta[2 * ntypes + pidx], coeff, pr.r);
       }
-      const R drho_j = spline_deriv<R>(smeta[ntypes + tj], coeff, r);
-      const R drho_i = spline_deriv<R>(smeta[ntypes + ti], coeff, r);
-      const R fpair = -(dphi + fpi * drho_j + fp[j] * drho_i) / r;
-      fx += static_cast<double>(fpair * dx);
-      fy += static_cast<double>(fpair * dy);
-      fz += static_cast<double>(fpair * dz);
+      const R drho_j = spline_deriv<R>(smeta[ntypes + tj], coeff, pr.r);
+      const R drho_i = spline_deriv<R>(smeta[ntypes + ti], coeff, pr.r);
+      const double fpair =
+          static_cast<double>(-(dphi + fpi * drho_j + fp[j] * drho_i) * pr.rinv);
+      fx = fma(fpair, dx, fx);
+      fy = fma(fpair, dy, fy);
+      fz = fma(fpair, dz, fz);
     }
     forces[3 * i] = static_cast<R>(fx);
     forces[3 * i + 1] = static_cast<R>(fy);
@@ -194,18 +186,15 @@ int morse_launch(const double* pos, void* forces, const int32_t* nbrs, const int
                  const tdmd_morse_params& p, double* d_energy, void* work, hipStream_t stream) {
   const int grid = grid_for(natoms);
   double* partials = static_cast<double*>(work);
+  const MorseCoef c{p.d, p.alpha, p.r0, p.rc_sq};
   if (d_energy) {
     morse_csr_kernel<R, true><<<grid, kBlock, 0, stream>>>(
-        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, static_cast<R>(p.d),
-        static_cast<R>(p.alpha), static_cast<R>(p.r0), static_cast<R>(p.rc_sq), partials,
-        d_energy);
+        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, c, partials, d_energy);
     TDMD_LAUNCH_CHECK();
     if (partials) return launch_sum_partials(partials, grid, d_energy, 1, stream);
   } else {
     morse_csr_kernel<R, false><<<grid, kBlock, 0, stream>>>(
-        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, static_cast<R>(p.d),
-        static_cast<R>(p.alpha), static_cast<R>(p.r0), static_cast<R>(p.rc_sq), nullptr,
-        nullptr);
+        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, c, nullptr, nullptr);
     TDMD_LAUNCH_CHECK();
   }
   return TDMD_OK;
@@ -218,8 +207,8 @@ int eam_launch(const tdmd_eam_tables* t, const double* pos, void* forces, const 
                hipStream_t stream) {
   using Q = typename Quad<R>::type;
   const int grid = grid_for(natoms);
-  const R rc_sq = static_cast<R>(t->cutoff * t->cutoff);
-  const auto* meta = static_cast<const SplineMeta<R>*>(t->d_meta);
+  const double rc_sq = t->cutoff * t->cutoff;
+  const auto* meta = static_cast<const SplineMeta*>(t->d_meta);
   const auto* coeff = static_cast<const Q*>(t->d_coeff);
   R* fp = static_cast<R*>(fpv);
   double* partials = static_cast<double*>(work);

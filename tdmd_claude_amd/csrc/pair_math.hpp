@@ -1,0 +1,131 @@
+// pair_math.hpp -- per-pair arithmetic shared by every force kernel.
+//
+// Precision contract (R = float for the mixed path, double for FP64):
+//   * pair difference, minimum image, r^2, the cutoff test and r itself are
+//     carried in FP64. On CDNA4 an FP64 FMA issues at the rate of an
+//     unpacked FP32 FMA, so this costs registers, not throughput; it removes
+//     the dominant mixed-precision error, rounding r to float (the
+//     reference casts the FP64 difference to float first,
+//     device_morse.cu:69-74).
+//   * spline coefficients, the cubic in the interval offset dx, the
+//     potential functions and fpair are computed in R; dx itself is formed
+//     in FP64 (r - knot) and then rounded, so the interval position keeps
+//     full precision.
+//   * force and density sums are FP64.
+// For R = float, r comes from a float square root refined by one FP64
+// Newton step (error ~1e-16 relative), avoiding the slow FP64 sqrt.
+#pragma once
+
+#include "tables.hpp"
+
+namespace tdmd_hip {
+
+template <class R>
+struct PairR {
+  double r;  // FP64 distance
+  R rinv;    // 1 / r in R
+};
+
+template <class R>
+__device__ __forceinline__ PairR<R> pair_r(double r2);
+
+template <>
+__device__ __forceinline__ PairR<double> pair_r<double>(double r2) {
+  PairR<double> p;
+  p.r = sqrt(r2);
+  p.rinv = 1.0 / p.r;
+  return p;
+}
+
+template <>
+__device__ __forceinline__ PairR<float> pair_r<float>(double r2) {
+  PairR<float> p;
+  const float rf = sqrtf(static_cast<float>(r2));
+  const double rd = static_cast<double>(rf);
+  p.rinv = 1.0f / rf;
+  p.r = fma(fma(-rd, rd, r2), static_cast<double>(0.5f * p.rinv), rd);
+  return p;
+}
+
+template <class R>
+struct SplineVal {
+  R f, df;
+};
+
+// Interval and offset: idx = floor((x - rmin) / dr) clamped to [0, n-2]
+// (spline_eval, device_eam.cu:36-50), with a stored reciprocal spacing; at a
+// knot the two neighbouring cubics agree in value and slope (C1 FD-Hermite),
+// so the choice there changes results only by rounding.
+template <class R>
+__device__ __forceinline__ int spline_locate(const SplineMeta& m, double x, R* dx) {
+  int idx = static_cast<int>((x - m.rmin) * m.inv_dr);
+  idx = idx < 0 ? 0 : idx;
+  idx = idx > m.n - 2 ? m.n - 2 : idx;
+  *dx = static_cast<R>(x - (m.rmin + static_cast<double>(idx) * m.dr));
+  return idx;
+}
+
+template <class R>
+__device__ __forceinline__ R spline_value(const SplineMeta& m,
+                                          const typename Quad<R>::type* __restrict__ q,
+                                          double x) {
+  R dx;
+  const int idx = spline_locate<R>(m, x, &dx);
+  const auto c = q[m.offset + idx];
+  return c.x + dx * (c.y + dx * (c.z + dx * c.w));
+}
+
+template <class R>
+__device__ __forceinline__ R spline_deriv(const SplineMeta& m,
+                                          const typename Quad<R>::type* __restrict__ q,
+                                          double x) {
+  R dx;
+  const int idx = spline_locate<R>(m, x, &dx);
+  const auto c = q[m.offset + idx];
+  return c.y + dx * (static_cast<R>(2) * c.z + dx * static_cast<R>(3) * c.w);
+}
+
+template <class R>
+__device__ __forceinline__ SplineVal<R> spline_both(const SplineMeta& m,
+                                                    const typename Quad<R>::type* __restrict__ q,
+                                                    double x) {
+  R dx;
+  const int idx = spline_locate<R>(m, x, &dx);
+  const auto c = q[m.offset + idx];
+  SplineVal<R> v;
+  v.f = c.x + dx * (c.y + dx * (c.z + dx * c.w));
+  v.df = c.y + dx * (static_cast<R>(2) * c.z + dx * static_cast<R>(3) * c.w);
+  return v;
+}
+
+// Morse pair (morse.hpp:37-55): energy and fpair = -dU/dr / r.
+struct MorseCoef {
+  double D, alpha, r0, rc_sq;
+};
+
+template <class R>
+__device__ __forceinline__ R morse_fpair(const MorseCoef& p, const PairR<R>& pr, R* energy);
+
+template <>
+__device__ __forceinline__ double morse_fpair<double>(const MorseCoef& p, const PairR<double>& pr,
+                                                      double* energy) {
+  // the reference expression order, in FP64
+  const double e = exp(-p.alpha * (pr.r - p.r0));
+  const double ome = 1.0 - e;
+  *energy = p.D * (ome * ome - 1.0);
+  return -(2.0 * p.D * p.alpha * ome * e) * pr.rinv;
+}
+
+template <>
+__device__ __forceinline__ float morse_fpair<float>(const MorseCoef& p, const PairR<float>& pr,
+                                                    float* energy) {
+  // 1 - e from expm1 keeps the relative precision near r0, where e -> 1
+  const float D = static_cast<float>(p.D), a = static_cast<float>(p.alpha);
+  const float em1 = expm1f(-a * static_cast<float>(pr.r - p.r0));
+  const float e = 1.0f + em1;
+  const float ome = -em1;
+  *energy = D * (ome * ome - 1.0f);
+  return -(2.0f * D * a * ome * e) * pr.rinv;
+}
+
+}  // namespace tdmd_hip

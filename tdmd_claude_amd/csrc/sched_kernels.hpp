@@ -17,17 +17,16 @@
 //                         read of a wave is a single coalesced load
 //
 // Mirror encodings:
-//   MirrorF64    double4; pair difference in FP64, minimum image in FP64,
-//                then one cast to R (the reference's relative-coordinate
-//                trick, done before the minimum image so pairs that cross a
-//                periodic face keep full precision)
+//   MirrorF64    double4; pair difference and minimum image in FP64
 //   MirrorFixed  int4 of 32-bit fixed-point coordinates, unit 2^-s A with
-//                L 2^s < 2^31; the pair difference is exact integer
-//                arithmetic (minimum image in integers), then one
-//                int->float conversion. 16 bytes per gather instead of 32.
+//                L 2^s < 2^31; the pair difference and minimum image are
+//                exact integer arithmetic, converted exactly to FP64.
+//                16 bytes per gather instead of 32; positions are rounded
+//                to the 2^-s grid (2^-24 A for a 230 A box).
+// Either way the pair geometry downstream is FP64 (pair_math.hpp).
 #pragma once
 
-#include "spline_dev.hpp"
+#include "pair_math.hpp"
 
 namespace tdmd_hip {
 namespace sched {
@@ -38,7 +37,7 @@ constexpr int kMaxSplines = 64;
 struct MirrorParams {
   double lo[3];
   double q;       // 2^s (fixed) ; unused for F64
-  float inv_q;    // 2^-s
+  double inv_qd;  // 2^-s
   int Lq[3];      // round(L * 2^s)
   int hq[3];      // Lq / 2
   double L[3];
@@ -56,12 +55,13 @@ struct MirrorF64 {
     m.w = w;
     return m;
   }
-  template <class R>
-  __device__ __forceinline__ static void delta(const T& a, const T& b, const MirrorParams& mp,
-                                               R* dx, R* dy, R* dz) {
-    *dx = static_cast<R>(min_image(a.x - b.x, mp.L[0], mp.hL[0], mp.periodic[0]));
-    *dy = static_cast<R>(min_image(a.y - b.y, mp.L[1], mp.hL[1], mp.periodic[1]));
-    *dz = static_cast<R>(min_image(a.z - b.z, mp.L[2], mp.hL[2], mp.periodic[2]));
+  // Pair difference a - b with the minimum image, in FP64; returns r^2.
+  __device__ __forceinline__ static double delta(const T& a, const T& b, const MirrorParams& mp,
+                                                 double* dx, double* dy, double* dz) {
+    *dx = min_image(a.x - b.x, mp.L[0], mp.hL[0], mp.periodic[0]);
+    *dy = min_image(a.y - b.y, mp.L[1], mp.hL[1], mp.periodic[1]);
+    *dz = min_image(a.z - b.z, mp.L[2], mp.hL[2], mp.periodic[2]);
+    return *dx * *dx + *dy * *dy + *dz * *dz;
   }
   __device__ __forceinline__ static int type_of(const T& m) { return static_cast<int>(m.w); }
   template <class R>
@@ -98,13 +98,14 @@ struct MirrorFixed {
     }
     return d;
   }
-  template <class R>
-  __device__ __forceinline__ static void delta(const T& a, const T& b, const MirrorParams& mp,
-                                               R* dx, R* dy, R* dz) {
-    const R s = static_cast<R>(mp.inv_q);
-    *dx = static_cast<R>(imin_image(a.x - b.x, mp.Lq[0], mp.hq[0], mp.periodic[0])) * s;
-    *dy = static_cast<R>(imin_image(a.y - b.y, mp.Lq[1], mp.hq[1], mp.periodic[1])) * s;
-    *dz = static_cast<R>(imin_image(a.z - b.z, mp.Lq[2], mp.hq[2], mp.periodic[2])) * s;
+  // Exact integer difference and minimum image, converted exactly to FP64
+  // (|d| < 2^31 units of 2^-s A); returns r^2.
+  __device__ __forceinline__ static double delta(const T& a, const T& b, const MirrorParams& mp,
+                                                 double* dx, double* dy, double* dz) {
+    *dx = static_cast<double>(imin_image(a.x - b.x, mp.Lq[0], mp.hq[0], mp.periodic[0])) * mp.inv_qd;
+    *dy = static_cast<double>(imin_image(a.y - b.y, mp.Lq[1], mp.hq[1], mp.periodic[1])) * mp.inv_qd;
+    *dz = static_cast<double>(imin_image(a.z - b.z, mp.Lq[2], mp.hq[2], mp.periodic[2])) * mp.inv_qd;
+    return *dx * *dx + *dy * *dy + *dz * *dz;
   }
   __device__ __forceinline__ static int type_of(const T& m) { return m.w; }
   template <class R>

@@ -228,18 +228,17 @@ __global__ void k_nhc_commit(double* nhc) {
 // EAM passes
 // ---------------------------------------------------------------------------
 
-template <class R>
-__device__ __forceinline__ void load_meta(SplineMeta<R>* sm, const SplineMeta<R>* meta, int ns) {
+__device__ __forceinline__ void load_meta(SplineMeta* sm, const SplineMeta* meta, int ns) {
   for (int k = threadIdx.x; k < ns; k += blockDim.x) sm[k] = meta[k];
   __syncthreads();
 }
 
 template <class R, class Mir, bool kEnergy>
 __global__ void __launch_bounds__(kBlock)
-k_eam_density(StepArgs a, const SplineMeta<R>* __restrict__ meta,
-              const typename Quad<R>::type* __restrict__ coeff, R rc_sq) {
+k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
+              const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
   using T = typename Mir::T;
-  __shared__ SplineMeta<R> sm[kMaxSplines];
+  __shared__ SplineMeta sm[kMaxSplines];
   load_meta(sm, meta, 2 * a.ntypes);
   const int blk = xcd_block(a.nblocks);
   const int i = blk * kBlock + threadIdx.x;
@@ -253,15 +252,14 @@ k_eam_density(StepArgs a, const SplineMeta<R>* __restrict__ meta,
     for (int k = 0; k < n; ++k) {
       const int j = a.nbr[ell(i, k, a.cap)];
       const T mj = M[j];
-      R dx, dy, dz;
-      Mir::template delta<R>(mi, mj, a.mp, &dx, &dy, &dz);
-      const R r2 = dx * dx + dy * dy + dz * dz;
+      double dx, dy, dz;
+      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
       if (r2 < rc_sq) {
-        const R r = sqrt(r2);
-        rho += static_cast<double>(spline_value<R>(sm[a.ntypes + Mir::type_of(mj)], coeff, r));
+        const PairR<R> pr = pair_r<R>(r2);
+        rho += static_cast<double>(spline_value<R>(sm[a.ntypes + Mir::type_of(mj)], coeff, pr.r));
       }
     }
-    const SplineVal<R> F = spline_both<R>(sm[ti], coeff, static_cast<R>(rho));
+    const SplineVal<R> F = spline_both<R>(sm[ti], coeff, rho);
     static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
     e = static_cast<double>(F.f);
   }
@@ -270,10 +268,10 @@ k_eam_density(StepArgs a, const SplineMeta<R>* __restrict__ meta,
 
 template <class R, class Mir, int kEpi, bool kEnergy, bool kMulti>
 __global__ void __launch_bounds__(kBlock)
-k_eam_force(StepArgs a, const SplineMeta<R>* __restrict__ meta,
-            const typename Quad<R>::type* __restrict__ coeff, R rc_sq) {
+k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
+            const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
   using T = typename Mir::T;
-  __shared__ SplineMeta<R> sm[kMaxSplines];
+  __shared__ SplineMeta sm[kMaxSplines];
   const int nt = a.ntypes;
   load_meta(sm, meta, 2 * nt + nt * (nt + 1) / 2);
   const int blk = xcd_block(a.nblocks);
@@ -289,28 +287,27 @@ k_eam_force(StepArgs a, const SplineMeta<R>* __restrict__ meta,
     for (int k = 0; k < n; ++k) {
       const int j = a.nbr[ell(i, k, a.cap)];
       const T mj = M2[j];
-      R dx, dy, dz;
-      Mir::template delta<R>(mi, mj, a.mp, &dx, &dy, &dz);
-      const R r2 = dx * dx + dy * dy + dz * dz;
+      double dx, dy, dz;
+      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
       if (r2 < rc_sq) {
-        const R r = sqrt(r2);
+        const PairR<R> pr = pair_r<R>(r2);
         const int tj = kMulti ? a.types[j] - 1 : 0;
         const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
         R dphi;
         if (kEnergy) {
-          const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, r);
+          const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, pr.r);
           pe += 0.5 * static_cast<double>(ph.f);
           dphi = ph.df;
         } else {
-          dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, r);
+          dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, pr.r);
         }
-        const R drho_j = spline_deriv<R>(sm[nt + tj], coeff, r);
-        const R drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, r) : drho_j;
+        const R drho_j = spline_deriv<R>(sm[nt + tj], coeff, pr.r);
+        const R drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
         const R fpj = Mir::template w_real<R>(mj);
-        const R fpair = -(dphi + fpi * drho_j + fpj * drho_i) / r;
-        fx += static_cast<double>(fpair * dx);
-        fy += static_cast<double>(fpair * dy);
-        fz += static_cast<double>(fpair * dz);
+        const double fpair = static_cast<double>(-(dphi + fpi * drho_j + fpj * drho_i) * pr.rinv);
+        fx = fma(fpair, dx, fx);
+        fy = fma(fpair, dy, fy);
+        fz = fma(fpair, dz, fz);
       }
     }
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
@@ -320,23 +317,17 @@ k_eam_force(StepArgs a, const SplineMeta<R>* __restrict__ meta,
 }
 
 // ---------------------------------------------------------------------------
-// Morse pass (morse.hpp:37-55 pair math in R, FP64 per-atom sums)
+// Morse pass (morse.hpp:37-55; pair_math.hpp precision contract)
 // ---------------------------------------------------------------------------
 
-struct MorseR {
-  double D, alpha, r0, rc_sq;
-};
-
 template <class R, class Mir, int kEpi, bool kEnergy>
-__global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseR p) {
+__global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p) {
   // The Morse pass gathers from M, which the kick-drift epilogue rewrites.
   static_assert(kEpi != kFullKickDrift, "Morse kick-drift runs as its own launch");
   using T = typename Mir::T;
   const int blk = xcd_block(a.nblocks);
   const int i = blk * kBlock + threadIdx.x;
   const T* __restrict__ M = static_cast<const T*>(a.M);
-  const R D = static_cast<R>(p.D), alpha = static_cast<R>(p.alpha), r0 = static_cast<R>(p.r0),
-          rc_sq = static_cast<R>(p.rc_sq);
   double red = 0.0;
   if (i < a.natoms) {
     const T mi = M[i];
@@ -345,19 +336,16 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseR p) {
     for (int k = 0; k < n; ++k) {
       const int j = a.nbr[ell(i, k, a.cap)];
       const T mj = M[j];
-      R dx, dy, dz;
-      Mir::template delta<R>(mi, mj, a.mp, &dx, &dy, &dz);
-      const R r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < rc_sq) {
-        const R r = sqrt(r2);
-        const R e = exp(-alpha * (r - r0));
-        const R ome = static_cast<R>(1) - e;
-        if (kEnergy)
-          pe += static_cast<double>(static_cast<R>(0.5) * D * (ome * ome - static_cast<R>(1)));
-        const R fpair = -(static_cast<R>(2) * D * alpha * ome * e) / r;
-        fx += static_cast<double>(fpair * dx);
-        fy += static_cast<double>(fpair * dy);
-        fz += static_cast<double>(fpair * dz);
+      double dx, dy, dz;
+      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      if (r2 < p.rc_sq) {
+        const PairR<R> pr = pair_r<R>(r2);
+        R e;
+        const double fpair = static_cast<double>(morse_fpair<R>(p, pr, &e));
+        if (kEnergy) pe += 0.5 * static_cast<double>(e);
+        fx = fma(fpair, dx, fx);
+        fy = fma(fpair, dy, fy);
+        fz = fma(fpair, dz, fz);
       }
     }
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);

@@ -116,13 +116,11 @@ struct Engine final : EngineBase {
   explicit Engine(tdmd_sched* s) : S(s) {}
 
   using Q = typename Quad<R>::type;
-  const SplineMeta<R>* meta() const {
-    return static_cast<const SplineMeta<R>*>(S->tables->d_meta);
-  }
+  const SplineMeta* meta() const { return static_cast<const SplineMeta*>(S->tables->d_meta); }
   const Q* coeff() const { return static_cast<const Q*>(S->tables->d_coeff); }
-  R rc_sq() const { return static_cast<R>(S->tables->cutoff * S->tables->cutoff); }
-  MorseR mp() const {
-    return MorseR{S->morse.d, S->morse.alpha, S->morse.r0, S->morse.rc_sq};
+  double rc_sq() const { return S->tables->cutoff * S->tables->cutoff; }
+  MorseCoef mp() const {
+    return MorseCoef{S->morse.d, S->morse.alpha, S->morse.r0, S->morse.rc_sq};
   }
 
   template <int kEpi, bool kEnergy>
@@ -334,6 +332,9 @@ int sched_alloc(tdmd_sched* S) {
   TDMD_HIP_TRY(hipMemset(a.flags, 0, 32));
   TDMD_HIP_TRY(hipMemset(S->d_rebuilds, 0, 8));
   TDMD_HIP_TRY(hipMemset(a.nhc, 0, 64 * 8));
+  const double one = 1.0;  // no pending chain scale
+  TDMD_HIP_TRY(hipMemcpy(a.nhc + 24, &one, 8, hipMemcpyHostToDevice));
+  TDMD_HIP_TRY(hipDeviceSynchronize());
   return TDMD_OK;
 }
 
@@ -364,7 +365,7 @@ int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched
   int s = static_cast<int>(std::floor(std::log2(2147483647.0 / (Lmax * 1.001))));
   s = std::min(s, 60);
   mp.q = std::ldexp(1.0, s);
-  mp.inv_q = static_cast<float>(std::ldexp(1.0, -s));
+  mp.inv_qd = std::ldexp(1.0, -s);
   for (int d = 0; d < 3; ++d) {
     mp.Lq[d] = static_cast<int>(std::llround(mp.L[d] * mp.q));
     mp.hq[d] = mp.Lq[d] / 2;
@@ -380,6 +381,10 @@ int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched
   }
   g.ncells = g.n[0] * g.n[1] * g.n[2];
   S->a.natoms = natoms;
+  S->a.ntypes = S->tables ? S->tables->ntypes : 1;
+  if (S->a.ntypes < 1 || S->a.ntypes > kMaxTypes ||
+      2 * S->a.ntypes + S->a.ntypes * (S->a.ntypes + 1) / 2 > kMaxSplines)
+    return fail(TDMD_E_INVALID, "scheduler: 1..8 element types");
   S->a.nblocks = S->nblocks;
   S->a.dt = cfg.dt;
   S->a.skin_half_sq = 0.25 * cfg.r_skin * cfg.r_skin;
@@ -504,14 +509,11 @@ int compute_initial_forces(tdmd_sched* S) {
   }
   if ((rc = S->eng->launch(OP_F_ONLY, s))) return rc;
   if (S->nhcp.M > 0) {
-    // Stored KE of the uploaded velocities; no pending chain scale.
+    // Stored KE of the uploaded velocities (the chain state was reset by
+    // upload, with no pending scale).
     k_ke_partials<double><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a);
     k_nhc_set_ke<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, grid_for(S->natoms), 0);
     TDMD_LAUNCH_CHECK();
-    std::vector<double> init(64, 0.0);
-    init[24] = 1.0;
-    TDMD_HIP_TRY(hipMemcpyAsync(S->a.nhc, init.data(), 24 * 8, hipMemcpyHostToDevice, s));
-    TDMD_HIP_TRY(hipMemcpyAsync(S->a.nhc + 24, init.data() + 24, 8, hipMemcpyHostToDevice, s));
   }
   S->initial_forces = true;
   return TDMD_OK;
@@ -685,6 +687,10 @@ int tdmd_sched_upload(tdmd_sched* S, const double* pos, const double* vel, const
   std::iota(iota.begin(), iota.end(), 0);
   TDMD_HIP_TRY(hipMemcpy(S->g.orig, iota.data(), N * 4, hipMemcpyHostToDevice));
   TDMD_HIP_TRY(hipMemset(S->a.flags, 0, 32));
+  std::vector<double> nhc0(64, 0.0);  // fresh chain, no pending scale
+  nhc0[24] = 1.0;
+  TDMD_HIP_TRY(hipMemcpy(S->a.nhc, nhc0.data(), 64 * 8, hipMemcpyHostToDevice));
+  TDMD_HIP_TRY(hipDeviceSynchronize());
   const int rc = S->eng->write_mirror(S->stream);
   if (rc) return rc;
   TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
@@ -770,6 +776,51 @@ int tdmd_sched_interaction_cutoff(const tdmd_sched* S, double* rc) {
   if (!S || !rc) return fail(TDMD_E_INVALID, "sched_interaction_cutoff: null");
   *rc = S->cutoff;
   return TDMD_OK;
+}
+
+int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
+  // A window of `steps` NVE steps with HIP events around each pass of the
+  // steady-state step: EAM -> [density, force + kick-drift], Morse ->
+  // [kick-drift, force]. No rebuild test inside the window.
+  if (!S || !ms_out || steps < 2) return fail(TDMD_E_INVALID, "sched_time_passes: bad args");
+  if (!S->uploaded) return fail(TDMD_E_STATE, "sched_time_passes: upload() first");
+  if (S->nhcp.M > 0) return fail(TDMD_E_STATE, "sched_time_passes: NVE schedulers only");
+  int rc;
+  if (!S->initial_forces && (rc = compute_initial_forces(S))) return rc;
+  hipStream_t s = S->stream;
+  const bool eam = S->kind == 1;
+  const char a_op = eam ? OP_DENS : OP_KD_FULL;
+  const char b_op = eam ? OP_F_FULL : OP_F_ONLY;
+  std::vector<hipEvent_t> ev(static_cast<size_t>(3 * steps));
+  for (auto& e : ev) TDMD_HIP_TRY(hipEventCreate(&e));
+  if ((rc = S->eng->launch(OP_BOOT, s))) return rc;
+  if (!eam && (rc = S->eng->launch(OP_F_ONLY, s))) return rc;
+  for (int k = 0; k < steps - 1; ++k) {
+    TDMD_HIP_TRY(hipEventRecord(ev[3 * k], s));
+    if ((rc = S->eng->launch(a_op, s))) return rc;
+    TDMD_HIP_TRY(hipEventRecord(ev[3 * k + 1], s));
+    if ((rc = S->eng->launch(b_op, s))) return rc;
+    TDMD_HIP_TRY(hipEventRecord(ev[3 * k + 2], s));
+  }
+  if (eam && (rc = S->eng->launch(OP_DENS, s))) return rc;
+  if (!eam && (rc = S->eng->launch(OP_KD_FULL, s))) return rc;
+  if ((rc = S->eng->launch(OP_F_HALF, s))) return rc;
+  TDMD_HIP_TRY(hipStreamSynchronize(s));
+  double ta = 0, tb = 0;
+  for (int k = 0; k < steps - 1; ++k) {
+    float x, y;
+    TDMD_HIP_TRY(hipEventElapsedTime(&x, ev[3 * k], ev[3 * k + 1]));
+    TDMD_HIP_TRY(hipEventElapsedTime(&y, ev[3 * k + 1], ev[3 * k + 2]));
+    ta += x;
+    tb += y;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  ms_out[0] = static_cast<float>(ta / (steps - 1));
+  ms_out[1] = static_cast<float>(tb / (steps - 1));
+  S->current_step += steps;
+  S->steps_since += steps;
+  S->ticks += steps;
+  return check_flags(S);
 }
 
 int tdmd_sched_stream(const tdmd_sched* S, void** stream) {

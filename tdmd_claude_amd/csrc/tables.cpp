@@ -101,22 +101,22 @@ static int upload_quads(tdmd_eam_tables* t) {
     q[k].w = static_cast<R>(t->d[k]);
   }
   const int nt = t->ntypes, np = nt * (nt + 1) / 2, ns = 2 * nt + np;
-  std::vector<SplineMeta<R>> meta(static_cast<size_t>(ns));
+  std::vector<SplineMeta> meta(static_cast<size_t>(ns));
   for (int sidx = 0; sidx < ns; ++sidx) {
     const bool is_f = sidx < nt;
-    SplineMeta<R>& m = meta[static_cast<size_t>(sidx)];
+    SplineMeta& m = meta[static_cast<size_t>(sidx)];
     const double dr = is_f ? t->drho : t->dr;
-    m.dr = static_cast<R>(dr);
-    m.inv_dr = static_cast<R>(1.0 / dr);
-    m.rmin = static_cast<R>(0);
+    m.dr = dr;
+    m.inv_dr = 1.0 / dr;
+    m.rmin = 0.0;
     m.n = is_f ? t->nrho : t->nr;
     m.offset = t->offsets[static_cast<size_t>(sidx)];
   }
   t->coeff_bytes = nq * sizeof(Q);
   TDMD_HIP_TRY(hipMalloc(&t->d_coeff, t->coeff_bytes));
-  TDMD_HIP_TRY(hipMalloc(&t->d_meta, meta.size() * sizeof(SplineMeta<R>)));
+  TDMD_HIP_TRY(hipMalloc(&t->d_meta, meta.size() * sizeof(SplineMeta)));
   TDMD_HIP_TRY(hipMemcpy(t->d_coeff, q.data(), t->coeff_bytes, hipMemcpyHostToDevice));
-  TDMD_HIP_TRY(hipMemcpy(t->d_meta, meta.data(), meta.size() * sizeof(SplineMeta<R>),
+  TDMD_HIP_TRY(hipMemcpy(t->d_meta, meta.data(), meta.size() * sizeof(SplineMeta),
                          hipMemcpyHostToDevice));
   return TDMD_OK;
 }

@@ -13,13 +13,13 @@ namespace tdmd_hip {
 void build_fd_hermite(const double* v, int n, double dr, double* a, double* b, double* c,
                       double* d);
 
-// Device spline descriptor (DeviceSpline, src/potentials/device_eam.cuh:14-20),
-// plus the reciprocal spacing so lookups multiply instead of divide.
-template <class R>
+// Device spline descriptor (DeviceSpline, src/potentials/device_eam.cuh:14-20):
+// spacing, its reciprocal (lookups multiply instead of divide) and origin in
+// FP64 whatever the coefficient precision, so the interval offset is exact.
 struct SplineMeta {
-  R dr;
-  R inv_dr;
-  R rmin;
+  double dr;
+  double inv_dr;
+  double rmin;
   int n;       // knots; intervals = n - 1
   int offset;  // first interval in the coefficient array
 };
@@ -65,7 +65,7 @@ struct tdmd_eam_tables {
   double cutoff = 0;
   // Spline order in the coefficient array: F[0..nt), rho[0..nt), phi[0..np).
   void* d_coeff = nullptr;  // float4 / double4 quads
-  void* d_meta = nullptr;   // SplineMeta<R>[2*nt + np]
+  void* d_meta = nullptr;   // SplineMeta[2*nt + np]
   size_t coeff_bytes = 0;
   // FP64 coefficient copies on the host (for checks and the adjoint).
   std::vector<double> a, b, c, d;

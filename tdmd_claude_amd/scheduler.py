@@ -1,0 +1,183 @@
+"""Python mirror of FastPipelineScheduler over the C-ABI.
+
+Same names and meaning as the reference C++ class
+(src/scheduler/fast_pipeline_scheduler.cuh:36-192): construct with a box, an
+atom count and a potential (Morse parameters or EAM/alloy tables), upload,
+run_until(step), download in upload order, stats(). Errors raise
+``capi.TdmdError`` (the reference aborts or throws).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import enum
+
+import numpy as np
+
+from . import capi
+from .inputs import SetflTables, System
+
+
+class PotentialKind(enum.IntEnum):
+    Morse = 0
+    Eam = 1
+
+
+@dataclasses.dataclass
+class FastPipelineConfig:
+    """FastPipelineConfig (fast_pipeline_scheduler.cuh:42-46) plus precision,
+    rebuild policy, NVT and launch options."""
+
+    dt: float = 0.001
+    r_skin: float = 1.0
+    rebuild_every: int = 10
+    precision: str = "mixed"  # "mixed" | "fp64"
+    rebuild_policy: str = "fixed"  # "fixed" | "displacement"
+    t_target: float = 0.0  # > 0: NVT (Nose-Hoover chain)
+    t_period: float = 0.1
+    nhc_length: int = 3
+    use_graph: bool = True
+    mirror_fp64: bool = False
+    max_neighbors: int = 0
+
+    def to_c(self) -> capi.SchedConfig:
+        c = capi.SchedConfig()
+        capi.lib().tdmd_sched_config_default(C.byref(c))
+        c.dt = self.dt
+        c.r_skin = self.r_skin
+        c.rebuild_every = self.rebuild_every
+        c.precision = capi.TDMD_MIXED if self.precision == "mixed" else capi.TDMD_FP64
+        c.rebuild_policy = (capi.REBUILD_DISPLACEMENT if self.rebuild_policy == "displacement"
+                            else capi.REBUILD_FIXED)
+        c.t_target = self.t_target
+        c.t_period = self.t_period
+        c.nhc_length = self.nhc_length
+        c.use_graph = int(self.use_graph)
+        c.mirror_fp64 = int(self.mirror_fp64)
+        c.max_neighbors = self.max_neighbors
+        return c
+
+
+@dataclasses.dataclass
+class FastPipelineStats:
+    """FastPipelineStats (fast_pipeline_scheduler.cuh:49-53) + graph launches."""
+
+    ticks: int
+    kernel_launches: int
+    rebuilds: int
+    graph_launches: int
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pi(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class FastPipelineScheduler:
+    """One-GPU step scheduler; potential chosen by the constructor argument."""
+
+    def __init__(self, lo, hi, natoms: int, potential, cfg: FastPipelineConfig | None = None,
+                 periodic=(1, 1, 1)):
+        self.cfg = cfg or FastPipelineConfig()
+        self.natoms = int(natoms)
+        self._box = capi.Box.make(lo, hi, periodic)
+        h = C.c_void_p()
+        c = self.cfg.to_c()
+        if isinstance(potential, SetflTables):
+            t = potential
+            self._F = np.ascontiguousarray(t.F, dtype=np.float64)
+            self._rho = np.ascontiguousarray(t.rho, dtype=np.float64)
+            self._rphi = np.ascontiguousarray(t.rphi, dtype=np.float64)
+            capi.call("tdmd_sched_create_eam", C.byref(h), C.byref(self._box), self.natoms,
+                      t.ntypes, t.nrho, t.drho, t.nr, t.dr, t.cutoff, _pd(self._F),
+                      _pd(self._rho), _pd(self._rphi), C.byref(c))
+            self.kind = PotentialKind.Eam
+        else:
+            d, alpha, r0, rc = potential
+            p = capi.MorseParams.make(d, alpha, r0, rc)
+            capi.call("tdmd_sched_create_morse", C.byref(h), C.byref(self._box), self.natoms,
+                      C.byref(p), C.byref(c))
+            self.kind = PotentialKind.Morse
+        self._h = h
+
+    @classmethod
+    def for_system(cls, sys: System, potential, cfg: FastPipelineConfig | None = None):
+        s = cls(sys.lo, sys.hi, sys.natoms, potential, cfg, sys.periodic)
+        s.upload_system(sys)
+        return s
+
+    def upload(self, pos, vel, forces, types, ids, masses) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        masses = np.ascontiguousarray(masses, dtype=np.float64)
+        f = None if forces is None else np.ascontiguousarray(forces, dtype=np.float64)
+        capi.call("tdmd_sched_upload", self._h, _pd(pos), _pd(vel),
+                  None if f is None else _pd(f), _pi(types), _pi(ids), _pd(masses), len(masses))
+
+    def upload_system(self, sys: System) -> None:
+        self.upload(sys.pos, sys.vel, None, sys.types, sys.ids, sys.masses)
+
+    def run_until(self, target_step: int) -> None:
+        capi.call("tdmd_sched_run_until", self._h, int(target_step))
+
+    def download(self):
+        n = self.natoms
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        f = np.zeros((n, 3))
+        types = np.zeros(n, dtype=np.int32)
+        ids = np.zeros(n, dtype=np.int32)
+        capi.call("tdmd_sched_download", self._h, _pd(pos), _pd(vel), _pd(f), _pi(types),
+                  _pi(ids), n)
+        return pos, vel, f, types, ids
+
+    def stats(self) -> FastPipelineStats:
+        out = (C.c_int64 * 4)()
+        capi.call("tdmd_sched_stats", self._h, out)
+        return FastPipelineStats(*[int(x) for x in out])
+
+    def current_step(self) -> int:
+        s = C.c_int32()
+        capi.call("tdmd_sched_current_step", self._h, C.byref(s))
+        return s.value
+
+    def potential_kind(self) -> PotentialKind:
+        return self.kind
+
+    def interaction_cutoff(self) -> float:
+        rc = C.c_double()
+        capi.call("tdmd_sched_interaction_cutoff", self._h, C.byref(rc))
+        return rc.value
+
+    def thermo(self) -> dict:
+        out = (C.c_double * 6)()
+        capi.call("tdmd_sched_thermo", self._h, out)
+        return dict(pe=out[0], ke=out[1], nhc=out[2], temp=out[3], max_nbr=out[4],
+                    mean_nbr=out[5])
+
+    def time_passes(self, steps: int = 20) -> tuple[float, float]:
+        """Mean ms of the two passes of a steady-state NVE step (advances state)."""
+        out = (C.c_float * 2)()
+        capi.call("tdmd_sched_time_passes", self._h, int(steps), out)
+        return float(out[0]), float(out[1])
+
+    def stream(self) -> C.c_void_p:
+        s = C.c_void_p()
+        capi.call("tdmd_sched_stream", self._h, C.byref(s))
+        return s
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            capi.lib().tdmd_sched_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
