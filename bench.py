@@ -136,6 +136,13 @@ def main() -> int:
     log(f"system: {N} atoms, box {system.hi[0]:.2f} A")
 
     if args.profile:
+        # a plain copy kernel of known size calibrates the HBM byte counters
+        nwords = 1 << 28  # 1 GiB
+        src, dst = capi.DeviceArray(nwords, np.uint32), capi.DeviceArray(nwords, np.uint32)
+        capi.call("tdmd_copy_u32", dst.ptr, src.ptr, nwords, None)
+        capi.call("tdmd_device_synchronize")
+        src.free()
+        dst.free()
         sched = FastPipelineScheduler.for_system(
             system, tables, FastPipelineConfig(**{**nvt_cfg("mixed").__dict__,
                                                   "mirror_fp64": args.mirror_fp64}))

@@ -66,6 +66,9 @@ int tdmd_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
 int tdmd_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 /* Kernel fill (graph-capture safe); DeviceBuffer::zero :103-107. */
 int tdmd_fill_u32(void* dst, uint32_t value, size_t count, void* stream);
+/* Kernel copy of `count` 32-bit words (DeviceBuffer device-to-device copy;
+ * also the known-size kernel that calibrates HBM byte counters). */
+int tdmd_copy_u32(void* dst, const void* src, size_t count, void* stream);
 int tdmd_stream_create(void** stream);
 int tdmd_stream_destroy(void* stream);
 int tdmd_stream_synchronize(void* stream);

@@ -105,6 +105,7 @@ _PROTOS = {
     "tdmd_memcpy_dtoh": (C.c_int, [_VP, _VP, _SZ, _VP]),
     "tdmd_memcpy_dtod": (C.c_int, [_VP, _VP, _SZ, _VP]),
     "tdmd_fill_u32": (C.c_int, [_VP, C.c_uint32, _SZ, _VP]),
+    "tdmd_copy_u32": (C.c_int, [_VP, _VP, _SZ, _VP]),
     "tdmd_stream_create": (C.c_int, [C.POINTER(_VP)]),
     "tdmd_stream_destroy": (C.c_int, [_VP]),
     "tdmd_stream_synchronize": (C.c_int, [_VP]),

@@ -33,6 +33,13 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v,
   for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ void copy_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                size_t n) {
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void sum_partials_kernel(const double* __restrict__ partials, int n,
                                     double* __restrict__ out, int accumulate) {
   // One block of 256 threads: strided per-thread sums in fixed order, then
@@ -138,6 +145,15 @@ int tdmd_fill_u32(void* dst, uint32_t value, size_t count, void* stream) {
   int grid = static_cast<int>((count + kBlock - 1) / kBlock);
   if (grid > 4096) grid = 4096;
   fill_u32_kernel<<<grid, kBlock, 0, as_stream(stream)>>>(static_cast<uint32_t*>(dst), value, count);
+  TDMD_LAUNCH_CHECK();
+  return TDMD_OK;
+}
+
+int tdmd_copy_u32(void* dst, const void* src, size_t count, void* stream) {
+  if (count == 0) return TDMD_OK;
+  if (!dst || !src) return fail(TDMD_E_INVALID, "tdmd_copy_u32: null pointer");
+  copy_u32_kernel<<<2048, kBlock, 0, as_stream(stream)>>>(
+      static_cast<uint32_t*>(dst), static_cast<const uint32_t*>(src), count);
   TDMD_LAUNCH_CHECK();
   return TDMD_OK;
 }

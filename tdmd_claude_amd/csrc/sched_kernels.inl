@@ -47,8 +47,9 @@ __device__ __forceinline__ R* rp(void* p) {
   return static_cast<R*>(p);
 }
 
+template <int BLOCK = kBlock>
 __device__ __forceinline__ void emit_partial(double v, double* partials, int slot) {
-  const double t = block_sum<kBlock>(v);
+  const double t = block_sum<BLOCK>(v);
   if (threadIdx.x == 0) partials[slot] = t;
 }
 

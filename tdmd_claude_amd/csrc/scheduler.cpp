@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "sched_kernels.inl"
+#include "sched_lds.inl"
 
 namespace tdmd_hip {
 
@@ -68,6 +69,7 @@ uint32_t spread3(uint32_t v) {
 // ---------------------------------------------------------------------------
 struct EngineBase {
   virtual ~EngineBase() = default;
+  virtual int init() = 0;
   virtual int launch(char op, hipStream_t s) = 0;
   virtual int write_mirror(hipStream_t s) = 0;
   virtual int energy_pass(hipStream_t s, double* d_out) = 0;  // d_out[0..2]
@@ -82,7 +84,8 @@ struct tdmd_sched {
   int kind = 0;  // 0 Morse, 1 EAM
   tdmd_box box{};
   int natoms = 0;
-  int nblocks = 0;
+  int nblocks = 0;   // 256-thread grid (multiple of 8)
+  int nb_force = 0;  // grid of the EAM/Morse passes = block partials they emit
   tdmd_sched_config cfg{};
   tdmd_morse_params morse{};
   tdmd_eam_tables* tables = nullptr;
@@ -123,11 +126,80 @@ struct Engine final : EngineBase {
     return MorseCoef{S->morse.d, S->morse.alpha, S->morse.r0, S->morse.rc_sq};
   }
 
+  // Single-element EAM runs the LDS-table kernels (sched_lds.inl); the
+  // windows are sized once from the device's LDS limit.
+  bool lds = false;
+  LdsWindow w_dens[2]{}, w_force[2]{};  // [kEnergy]
+  size_t b_dens[2]{}, b_force[2]{};
+
+  static LdsWindow window(int nint, size_t per, size_t budget, size_t* bytes) {
+    LdsWindow w;
+    w.nwin = static_cast<int>(std::min<size_t>(static_cast<size_t>(nint), budget / per));
+    w.k_lo = nint - w.nwin;
+    *bytes = static_cast<size_t>(w.nwin) * per;
+    return w;
+  }
+  template <class K>
+  static int allow_lds(K* fn, size_t bytes) {
+    TDMD_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(bytes)));
+    return TDMD_OK;
+  }
+
+  int init() override {
+    S->nb_force = S->nblocks;
+    if (S->kind != 1 || S->tables->ntypes != 1) return TDMD_OK;
+    // rho_0 and phi_00 must share the r grid (setfl: one Nr, dr).
+    SplineMeta m[3];
+    TDMD_HIP_TRY(hipMemcpy(m, S->tables->d_meta, sizeof(m), hipMemcpyDeviceToHost));
+    if (m[1].n != m[2].n || m[1].dr != m[2].dr || m[1].rmin != m[2].rmin || m[1].n < 2)
+      return TDMD_OK;
+    int dev = 0, lmax = 0;
+    TDMD_HIP_TRY(hipGetDevice(&dev));
+    TDMD_HIP_TRY(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    // The dynamic window plus the kernels' static LDS (block reduction)
+    // must fit the per-workgroup limit; keep 1 KiB for the static part.
+    const size_t budget = static_cast<size_t>(std::min(lmax, 160 * 1024) - 1024);
+    const int nint = m[1].n - 1;
+    using V2 = typename Vec2<R>::type;
+    for (int e = 0; e < 2; ++e) {
+      w_dens[e] = window(nint, sizeof(Q), budget, &b_dens[e]);
+      w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0), budget, &b_force[e]);
+    }
+    int rc = 0;
+    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false>, b_dens[0])) ||
+        (rc = allow_lds(&k_eam_density_lds<R, Mir, true>, b_dens[1])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true>, b_force[1])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false>, b_force[0])))
+      return rc;
+    lds = true;
+    S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
+    return TDMD_OK;
+  }
+
+  template <bool kEnergy>
+  void density(hipStream_t s) {
+    if (lds) {
+      const int nb = S->nb_force;
+      k_eam_density_lds<R, Mir, kEnergy><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
+          S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
+    } else {
+      k_eam_density<R, Mir, kEnergy><<<S->nblocks, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+    }
+  }
+
   template <int kEpi, bool kEnergy>
   void force(hipStream_t s) {
     const int nb = S->nblocks;
     if (S->kind == 1) {
-      if (S->tables->ntypes > 1)
+      if (lds)
+        k_eam_force_lds<R, Mir, kEpi, kEnergy><<<S->nb_force, kBigBlock, b_force[kEnergy], s>>>(
+            S->a, meta(), coeff(), rc_sq(), w_force[kEnergy], S->nb_force);
+      else if (S->tables->ntypes > 1)
         k_eam_force<R, Mir, kEpi, kEnergy, true><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
       else
         k_eam_force<R, Mir, kEpi, kEnergy, false><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
@@ -172,9 +244,9 @@ struct Engine final : EngineBase {
     return TDMD_OK;
   }
   int energy_pass(hipStream_t s, double* d_out) override {
-    const int nb = S->nblocks;
+    const int nb = S->nb_force;
     if (S->kind == 1) {
-      k_eam_density<R, Mir, true><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+      density<true>(s);
       launch_sum_partials(S->a.partials, nb, d_out + 0, 0, s);
     } else {
       tdmd_fill_u32(d_out, 0u, 2, s);
@@ -204,7 +276,7 @@ struct Engine final : EngineBase {
         k_nhc_pre<<<1, 64, 0, s>>>(S->a.nhc, S->nhcp);
         break;
       case OP_NVT_POST:
-        k_nhc_post<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, nb, S->nhcp);
+        k_nhc_post<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
         break;
       case OP_NVT_FINAL:
         k_scale_v<<<gb, kBlock, 0, s>>>(S->a);
@@ -222,7 +294,7 @@ struct Engine final : EngineBase {
         return rc ? rc : fill_pass(s);
       }
       case OP_DENS:
-        k_eam_density<R, Mir, false><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+        density<false>(s);
         break;
       case OP_F_FULL:
         force<kFullKickDrift, false>(s);
@@ -403,7 +475,7 @@ int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched
     S->eng.reset(new Engine<float, MirrorFixed>(S));
   else
     S->eng.reset(new Engine<float, MirrorF64>(S));
-  return TDMD_OK;
+  return S->eng->init();
 }
 
 void sched_free(tdmd_sched* S) {
