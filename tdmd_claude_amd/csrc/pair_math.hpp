@@ -39,10 +39,14 @@ __device__ __forceinline__ PairR<double> pair_r<double>(double r2) {
 
 template <>
 __device__ __forceinline__ PairR<float> pair_r<float>(double r2) {
+  // Hardware square root and reciprocal (v_sqrt_f32 / v_rcp_f32, about
+  // 1 ulp; pair distances are far from the denormal range), then one FP64
+  // Newton step r = s + (r2 - s^2) / (2 s): the step squares the seed's
+  // relative error, so r is accurate to ~1e-14 whichever seed is used.
   PairR<float> p;
-  const float rf = sqrtf(static_cast<float>(r2));
+  const float rf = __builtin_amdgcn_sqrtf(static_cast<float>(r2));
   const double rd = static_cast<double>(rf);
-  p.rinv = 1.0f / rf;
+  p.rinv = __builtin_amdgcn_rcpf(rf);
   p.r = fma(fma(-rd, rd, r2), static_cast<double>(0.5f * p.rinv), rd);
   return p;
 }

@@ -18,11 +18,14 @@
 //
 // Mirror encodings:
 //   MirrorF64    double4; pair difference and minimum image in FP64
-//   MirrorFixed  int4 of 32-bit fixed-point coordinates, unit 2^-s A with
-//                L 2^s < 2^31; the pair difference and minimum image are
-//                exact integer arithmetic, converted exactly to FP64.
+//   MirrorFixed  int4 of 32-bit fixed-point coordinates. On a periodic
+//                axis one box length is exactly 2^32 units, so the
+//                wrapping 32-bit difference of two coordinates IS the
+//                minimum-image difference (no compare, no select); on an
+//                open axis the unit is W/2^30 with W = max(L, r_list) and
+//                coordinates must stay inside [lo - 2W, lo + 2W).
 //                16 bytes per gather instead of 32; positions are rounded
-//                to the 2^-s grid (2^-24 A for a 230 A box).
+//                to the grid (L/2^32 = 5.4e-8 A for a 231 A box).
 // Either way the pair geometry downstream is FP64 (pair_math.hpp).
 #pragma once
 
@@ -36,14 +39,20 @@ constexpr int kMaxSplines = 64;
 
 struct MirrorParams {
   double lo[3];
-  double q;       // 2^s (fixed) ; unused for F64
-  double inv_qd;  // 2^-s
-  int Lq[3];      // round(L * 2^s)
-  int hq[3];      // Lq / 2
+  double q[3];     // fixed-point units per A (MirrorFixed)
+  double unit[3];  // A per unit, exact: L 2^-32 (periodic) or W 2^-30 (open)
   double L[3];
   double hL[3];
   int periodic[3];
 };
+
+// Minimum image as selects (no divergent branches); same values as
+// min_image() in common.hpp.
+__device__ __forceinline__ double min_image_sel(double d, double L, double hL, int periodic) {
+  const double up = d - L, dn = d + L;
+  const double m = d > hL ? up : (d < -hL ? dn : d);
+  return periodic ? m : d;
+}
 
 struct MirrorF64 {
   using T = double4;
@@ -58,10 +67,14 @@ struct MirrorF64 {
   // Pair difference a - b with the minimum image, in FP64; returns r^2.
   __device__ __forceinline__ static double delta(const T& a, const T& b, const MirrorParams& mp,
                                                  double* dx, double* dy, double* dz) {
-    *dx = min_image(a.x - b.x, mp.L[0], mp.hL[0], mp.periodic[0]);
-    *dy = min_image(a.y - b.y, mp.L[1], mp.hL[1], mp.periodic[1]);
-    *dz = min_image(a.z - b.z, mp.L[2], mp.hL[2], mp.periodic[2]);
+    *dx = min_image_sel(a.x - b.x, mp.L[0], mp.hL[0], mp.periodic[0]);
+    *dy = min_image_sel(a.y - b.y, mp.L[1], mp.hL[1], mp.periodic[1]);
+    *dz = min_image_sel(a.z - b.z, mp.L[2], mp.hL[2], mp.periodic[2]);
     return *dx * *dx + *dy * *dy + *dz * *dz;
+  }
+  __device__ __forceinline__ static bool representable(const MirrorParams&, double, double,
+                                                       double) {
+    return true;
   }
   __device__ __forceinline__ static int type_of(const T& m) { return static_cast<int>(m.w); }
   template <class R>
@@ -76,35 +89,45 @@ struct MirrorF64 {
 
 struct MirrorFixed {
   using T = int4;
+  // Coordinate in units, reduced mod 2^32 (a periodic image on a periodic
+  // axis; an open axis stays in range, see representable()).
+  __device__ __forceinline__ static int enc1(double x, double lo, double q) {
+    const long long v = __double2ll_rn((x - lo) * q);
+    return static_cast<int>(static_cast<unsigned int>(static_cast<unsigned long long>(v)));
+  }
   __device__ __forceinline__ static T encode(const MirrorParams& mp, double x, double y,
                                              double z, int w) {
     T m;
-    m.x = static_cast<int>(__double2ll_rn((x - mp.lo[0]) * mp.q));
-    m.y = static_cast<int>(__double2ll_rn((y - mp.lo[1]) * mp.q));
-    m.z = static_cast<int>(__double2ll_rn((z - mp.lo[2]) * mp.q));
+    m.x = enc1(x, mp.lo[0], mp.q[0]);
+    m.y = enc1(y, mp.lo[1], mp.q[1]);
+    m.z = enc1(z, mp.lo[2], mp.q[2]);
     m.w = w;
     return m;
+  }
+  __device__ __forceinline__ static bool rep1(double x, double lo, double q, int periodic) {
+    const double u = (x - lo) * q;
+    return periodic || (u >= -2147483648.0 && u < 2147483647.0);
+  }
+  __device__ __forceinline__ static bool representable(const MirrorParams& mp, double x,
+                                                       double y, double z) {
+    return rep1(x, mp.lo[0], mp.q[0], mp.periodic[0]) &&
+           rep1(y, mp.lo[1], mp.q[1], mp.periodic[1]) &&
+           rep1(z, mp.lo[2], mp.q[2], mp.periodic[2]);
   }
   __device__ __forceinline__ static T with_w(T m, int w) {
     m.w = w;
     return m;
   }
-  __device__ __forceinline__ static int imin_image(int d, int Lq, int hq, int periodic) {
-    if (periodic) {
-      if (d > hq)
-        d -= Lq;
-      else if (d < -hq)
-        d += Lq;
-    }
-    return d;
+  __device__ __forceinline__ static int wdiff(int a, int b) {
+    return static_cast<int>(static_cast<unsigned int>(a) - static_cast<unsigned int>(b));
   }
-  // Exact integer difference and minimum image, converted exactly to FP64
-  // (|d| < 2^31 units of 2^-s A); returns r^2.
+  // Wrapping 32-bit difference (the minimum image on a periodic axis),
+  // scaled to A in FP64; returns r^2.
   __device__ __forceinline__ static double delta(const T& a, const T& b, const MirrorParams& mp,
                                                  double* dx, double* dy, double* dz) {
-    *dx = static_cast<double>(imin_image(a.x - b.x, mp.Lq[0], mp.hq[0], mp.periodic[0])) * mp.inv_qd;
-    *dy = static_cast<double>(imin_image(a.y - b.y, mp.Lq[1], mp.hq[1], mp.periodic[1])) * mp.inv_qd;
-    *dz = static_cast<double>(imin_image(a.z - b.z, mp.Lq[2], mp.hq[2], mp.periodic[2])) * mp.inv_qd;
+    *dx = static_cast<double>(wdiff(a.x, b.x)) * mp.unit[0];
+    *dy = static_cast<double>(wdiff(a.y, b.y)) * mp.unit[1];
+    *dz = static_cast<double>(wdiff(a.z, b.z)) * mp.unit[2];
     return *dx * *dx + *dy * *dy + *dz * *dz;
   }
   __device__ __forceinline__ static int type_of(const T& m) { return m.w; }

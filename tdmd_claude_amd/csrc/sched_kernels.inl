@@ -79,7 +79,8 @@ __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vsca
   const double dy = min_image(y - a.by[i], a.mp.L[1], a.mp.hL[1], a.mp.periodic[1]);
   const double dz = min_image(z - a.bz[i], a.mp.L[2], a.mp.hL[2], a.mp.periodic[2]);
   if (dx * dx + dy * dy + dz * dz > a.skin_half_sq) a.flags[0] = 1;
-  if (!(isfinite(x) && isfinite(y) && isfinite(z))) a.flags[2] = 1;
+  if (!(isfinite(x) && isfinite(y) && isfinite(z) && Mir::representable(a.mp, x, y, z)))
+    a.flags[2] = 1;
 }
 
 // Force-pass epilogue.
@@ -557,7 +558,13 @@ k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, int32_t* nbr, int32_t* cnt
         }
       }
     }
-    if (!kCount) cnt[i] = count < a.cap ? count : a.cap;
+    if (!kCount) {
+      const int n = count < a.cap ? count : a.cap;
+      cnt[i] = n;
+      // Pad the row to a multiple of 4 with the atom itself, so unrolled
+      // readers can load four slots at a time (cap is a multiple of 4).
+      for (int k = n; k < ((n + 3) & ~3); ++k) nbr[ell(i, k, a.cap)] = i;
+    }
   }
   const int m = wave_max_i(count);
   if ((threadIdx.x & 63) == 0 && m > 0) {

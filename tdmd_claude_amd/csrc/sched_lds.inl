@@ -7,8 +7,13 @@
 // LDS; a pair closer than r_lo reads the interval from global memory instead
 // (same coefficients, same arithmetic). The window is sized to the LDS
 // budget: for the 5000-point tables of the benchmark it starts near 1 A
-// (mixed) and 1.8 A (FP64), below any distance a solid at a few hundred
-// kelvin reaches.
+// (mixed: in fact the whole table fits) and 1.8 A (FP64), below any
+// distance a solid at a few hundred kelvin reaches. kWin = false is the
+// specialisation for a window that holds every interval (no global path).
+//
+// Neighbour rows are padded to a multiple of 4 (with the atom itself), so
+// the four index loads of an unrolled group need no predicate; the padded
+// slots are excluded by k + u < n.
 //
 // Force window entry: {phi b, phi c, phi d, rho b} + {rho c, rho d}
 // (+ phi a when the energy is wanted); density window entry: rho {a,b,c,d}.
@@ -53,7 +58,7 @@ __device__ __forceinline__ int big_block_index(int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
-template <class R, class Mir, bool kEnergy>
+template <class R, class Mir, bool kEnergy, bool kWin>
 __global__ void __launch_bounds__(kBigBlock)
 k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
                   const typename Quad<R>::type* __restrict__ coeff, double rc_sq, LdsWindow w,
@@ -76,7 +81,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     for (int k = 0; k < n; k += kUnroll) {
       int j[kUnroll];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) j[u] = (k + u < n) ? a.nbr[ell(i, k + u, a.cap)] : i;
+      for (int u = 0; u < kUnroll; ++u) j[u] = a.nbr[ell(i, k + u, a.cap)];
       T mj[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) mj[u] = M[j[u]];
@@ -89,7 +94,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
           R t;
           const int idx = spline_locate<R>(mrho, pr.r, &t);
           const int q = idx - w.k_lo;
-          const Q c = q >= 0 ? tab[q] : coeff[mrho.offset + idx];
+          const Q c = (!kWin || q >= 0) ? tab[q] : coeff[mrho.offset + idx];
           rho += static_cast<double>(cubic_value<R>(c.x, c.y, c.z, c.w, t));
         }
       }
@@ -101,7 +106,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   if (kEnergy) emit_partial<kBigBlock>(e, a.partials, blk);
 }
 
-template <class R, class Mir, int kEpi, bool kEnergy>
+template <class R, class Mir, int kEpi, bool kEnergy, bool kWin>
 __global__ void __launch_bounds__(kBigBlock)
 k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
                 const typename Quad<R>::type* __restrict__ coeff, double rc_sq, LdsWindow w,
@@ -142,7 +147,7 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     for (int k = 0; k < n; k += kUnroll) {
       int j[kUnroll];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) j[u] = (k + u < n) ? a.nbr[ell(i, k + u, a.cap)] : i;
+      for (int u = 0; u < kUnroll; ++u) j[u] = a.nbr[ell(i, k + u, a.cap)];
       T mj[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) mj[u] = M2[j[u]];
@@ -156,7 +161,7 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
           const int idx = spline_locate<R>(mphi, pr.r, &t);
           const int q = idx - w.k_lo;
           R dphi, drho;
-          if (q >= 0) {
+          if (!kWin || q >= 0) {
             const Q x = tA[q];
             const V2 y = tB[q];
             dphi = cubic_deriv<R>(x.x, x.y, x.z, t);

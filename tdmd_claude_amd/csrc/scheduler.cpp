@@ -147,6 +147,20 @@ struct Engine final : EngineBase {
     return TDMD_OK;
   }
 
+  template <bool kWin>
+  int allow_all() {
+    int rc = 0;
+    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin>, b_dens[0])) ||
+        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin>, b_dens[1])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin>, b_force[1])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin>, b_force[0])))
+      return rc;
+    return TDMD_OK;
+  }
+
   int init() override {
     S->nb_force = S->nblocks;
     if (S->kind != 1 || S->tables->ntypes != 1) return TDMD_OK;
@@ -168,14 +182,7 @@ struct Engine final : EngineBase {
       w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0), budget, &b_force[e]);
     }
     int rc = 0;
-    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false>, b_dens[0])) ||
-        (rc = allow_lds(&k_eam_density_lds<R, Mir, true>, b_dens[1])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true>, b_force[1])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false>, b_force[0])))
-      return rc;
+    if ((rc = allow_all<false>()) || (rc = allow_all<true>())) return rc;
     lds = true;
     S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
     return TDMD_OK;
@@ -185,8 +192,13 @@ struct Engine final : EngineBase {
   void density(hipStream_t s) {
     if (lds) {
       const int nb = S->nb_force;
-      k_eam_density_lds<R, Mir, kEnergy><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
-          S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
+      const LdsWindow& w = w_dens[kEnergy];
+      if (w.k_lo > 0)
+        k_eam_density_lds<R, Mir, kEnergy, true><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
+            S->a, meta(), coeff(), rc_sq(), w, nb);
+      else
+        k_eam_density_lds<R, Mir, kEnergy, false><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
+            S->a, meta(), coeff(), rc_sq(), w, nb);
     } else {
       k_eam_density<R, Mir, kEnergy><<<S->nblocks, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
     }
@@ -196,9 +208,14 @@ struct Engine final : EngineBase {
   void force(hipStream_t s) {
     const int nb = S->nblocks;
     if (S->kind == 1) {
-      if (lds)
-        k_eam_force_lds<R, Mir, kEpi, kEnergy><<<S->nb_force, kBigBlock, b_force[kEnergy], s>>>(
-            S->a, meta(), coeff(), rc_sq(), w_force[kEnergy], S->nb_force);
+      if (lds && w_force[kEnergy].k_lo > 0)
+        k_eam_force_lds<R, Mir, kEpi, kEnergy, true>
+            <<<S->nb_force, kBigBlock, b_force[kEnergy], s>>>(S->a, meta(), coeff(), rc_sq(),
+                                                               w_force[kEnergy], S->nb_force);
+      else if (lds)
+        k_eam_force_lds<R, Mir, kEpi, kEnergy, false>
+            <<<S->nb_force, kBigBlock, b_force[kEnergy], s>>>(S->a, meta(), coeff(), rc_sq(),
+                                                               w_force[kEnergy], S->nb_force);
       else if (S->tables->ntypes > 1)
         k_eam_force<R, Mir, kEpi, kEnergy, true><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
       else
@@ -424,23 +441,21 @@ int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched
   S->nblocks = round_up(grid_for(natoms), 8);
   S->r_list = S->cutoff + cfg.r_skin;
   MirrorParams& mp = S->a.mp;
-  double Lmax = 0;
   for (int d = 0; d < 3; ++d) {
     mp.lo[d] = box.lo[d];
     mp.L[d] = box.hi[d] - box.lo[d];
     mp.hL[d] = 0.5 * mp.L[d];
     mp.periodic[d] = box.periodic[d] ? 1 : 0;
     if (!(mp.L[d] > 0)) return fail(TDMD_E_INVALID, "scheduler: box must have positive size");
-    Lmax = std::max(Lmax, mp.L[d]);
   }
-  // 32-bit fixed point: unit 2^-s with (1.001 Lmax) 2^s < 2^31.
-  int s = static_cast<int>(std::floor(std::log2(2147483647.0 / (Lmax * 1.001))));
-  s = std::min(s, 60);
-  mp.q = std::ldexp(1.0, s);
-  mp.inv_qd = std::ldexp(1.0, -s);
+  // 32-bit fixed point (MirrorFixed): a periodic axis spans exactly 2^32
+  // units; an open axis gets unit W 2^-30, W = max(L, r_list), so any listed
+  // pair difference stays far below 2^31 units.
   for (int d = 0; d < 3; ++d) {
-    mp.Lq[d] = static_cast<int>(std::llround(mp.L[d] * mp.q));
-    mp.hq[d] = mp.Lq[d] / 2;
+    const double W = mp.periodic[d] ? mp.L[d] : std::max(mp.L[d], S->r_list);
+    const int e = mp.periodic[d] ? 32 : 30;
+    mp.q[d] = std::ldexp(1.0, e) / W;
+    mp.unit[d] = std::ldexp(W, -e);
   }
   // Binning grid: cells no smaller than r_list (with a relative margin so
   // rounding in the bin computation can never push a neighbour out of the
@@ -491,7 +506,10 @@ void sched_free(tdmd_sched* S) {
 int check_flags(tdmd_sched* S) {
   int32_t f[4];
   TDMD_HIP_TRY(hipMemcpy(f, S->a.flags, 16, hipMemcpyDeviceToHost));
-  if (f[2]) return fail(TDMD_E_NONFINITE, "scheduler: non-finite position detected");
+  if (f[2])
+    return fail(TDMD_E_NONFINITE,
+                "scheduler: non-finite position, or an atom outside the representable range "
+                "of an open axis (lo - 2W .. lo + 2W, W = max(box length, r_list))");
   if (f[1] > S->a.cap)
     return fail(TDMD_E_OVERFLOW, "scheduler: neighbour capacity " + std::to_string(S->a.cap) +
                                      " exceeded (max count " + std::to_string(f[1]) + ")");
@@ -512,7 +530,7 @@ int first_build(tdmd_sched* S) {
   TDMD_HIP_TRY(hipStreamSynchronize(s));
   int cap = S->cfg.max_neighbors > 0 ? S->cfg.max_neighbors
                                      : round_up(static_cast<int>(std::ceil(maxc * 1.25)) + 8, 4);
-  cap = std::max(cap, 4);
+  cap = round_up(std::max(cap, 4), 4);  // rows are padded to multiples of 4
   if (S->a.nbr == nullptr || cap > S->a.cap) {
     if (S->a.nbr) TDMD_HIP_TRY(hipFree(const_cast<int32_t*>(S->a.nbr)));
     void* p = nullptr;
