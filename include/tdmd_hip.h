@@ -173,6 +173,15 @@ int tdmd_eam_tables_create(tdmd_eam_tables** out, int32_t precision,
                            int32_t nr, double dr, double cutoff,
                            const double* F, const double* rho,
                            const double* rphi);
+/* Same, from knot values as EamAlloy holds them after read_setfl
+ * (eam_alloy.cpp:152-191): F and rho as read, phi = (r phi)/r with
+ * phi[0] = 0, pair blocks in phi_index order (eam_alloy.hpp:88-92). This is
+ * what DeviceEam::upload_tables(const EamAlloy&) receives. */
+int tdmd_eam_tables_create_values(tdmd_eam_tables** out, int32_t precision,
+                                  int32_t ntypes, int32_t nrho, double drho,
+                                  int32_t nr, double dr, double cutoff,
+                                  const double* F, const double* rho,
+                                  const double* phi);
 int tdmd_eam_tables_destroy(tdmd_eam_tables* t);
 int tdmd_eam_tables_info(const tdmd_eam_tables* t, int32_t* ntypes,
                          double* cutoff);
@@ -258,7 +267,11 @@ typedef struct {
   int32_t mirror_fp64;    /* MIXED only: gather FP64 positions (1) or the */
                           /* 32-bit fixed-point mirror (0)                */
   int32_t max_neighbors;  /* 0 = sized from the first build               */
-  int32_t reserved[8];
+  int32_t pair_unroll;    /* neighbour slots per pipelined group: 0 (=8),  */
+                          /* 4 or 8 (single-element EAM passes)           */
+  int32_t lds_budget;     /* LDS bytes per workgroup for spline windows:  */
+                          /* 0 = device limit                             */
+  int32_t reserved[6];
 } tdmd_sched_config;
 
 void tdmd_sched_config_default(tdmd_sched_config* cfg);
@@ -272,6 +285,13 @@ int tdmd_sched_create_eam(tdmd_sched** out, const tdmd_box* box,
                           double drho, int32_t nr, double dr, double cutoff,
                           const double* F, const double* rho,
                           const double* rphi, const tdmd_sched_config* cfg);
+/* EAM from tables already on the device (the FastPipelineScheduler(box,
+ * natoms, const EamAlloy&, cfg) constructor after DeviceEam::upload_tables,
+ * fast_pipeline_scheduler.cu:27-40). The scheduler keeps its own copy in
+ * cfg->precision; `tables` stays owned by the caller. */
+int tdmd_sched_create_eam_tables(tdmd_sched** out, const tdmd_box* box,
+                                 int32_t natoms, const tdmd_eam_tables* tables,
+                                 const tdmd_sched_config* cfg);
 int tdmd_sched_destroy(tdmd_sched* s);
 /* upload (fast_pipeline_scheduler.cu:49-73). pos, vel, forces: double[3N];
  * masses: double[n_masses] indexed by 1-based type. forces may be NULL. */

@@ -33,6 +33,8 @@ SOURCES = [
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+ROCM = pathlib.Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 COMMON = [
     f"--offload-arch={ARCH}",
@@ -56,6 +58,33 @@ def _compile(src: str) -> pathlib.Path:
     return obj
 
 
+CPP_TEST = ROOT / "tests" / "cpp" / "dropin_test.cpp"
+CPP_BIN = ROOT / "tests" / "cpp" / "_bin"
+
+
+def build_cpp_tests(verbose: bool = True) -> list[pathlib.Path]:
+    """Compile the C++ drop-in test program once per precision macro. It is
+    host code over ``include/tdmd/`` and links the library by rpath."""
+    CPP_BIN.mkdir(parents=True, exist_ok=True)
+    out = []
+    for prec in ("MIXED", "FP64"):
+        exe = CPP_BIN / f"dropin_test_{prec.lower()}"
+        # hip_runtime_api.h for host compilers: the AMD platform macro and
+        # the ROCm include directory (what hipcc adds for host code).
+        cmd = [CXX, "-std=c++20", "-O2", "-Wall", f"-DTDMD_PRECISION_{prec}",
+               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", f"-I{ROOT / 'include'}",
+               str(CPP_TEST), f"-L{LIBDIR}", "-ltdmd_hip",
+               "-Wl,-rpath,$ORIGIN/../../../tdmd_claude_amd/lib", "-o", str(exe)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"C++ test build failed:\n{' '.join(cmd)}\n{res.stdout}\n"
+                               f"{res.stderr}")
+        out.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return out
+
+
 def build(verbose: bool = True) -> pathlib.Path:
     """Compile all sources and link the shared library. Always rebuilds."""
     if not pathlib.Path(HIPCC).exists() and shutil.which("hipcc") is None:
@@ -73,6 +102,7 @@ def build(verbose: bool = True) -> pathlib.Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_tests(verbose)
     return LIB
 
 

@@ -81,7 +81,9 @@ class SchedConfig(C.Structure):
         ("block_size", C.c_int32),
         ("mirror_fp64", C.c_int32),
         ("max_neighbors", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("pair_unroll", C.c_int32),
+        ("lds_budget", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -130,6 +132,8 @@ _PROTOS = {
                                   _PD]),
     "tdmd_eam_tables_create": (C.c_int, [C.POINTER(_VP), _I32, _I32, _I32, _D, _I32, _D, _D,
                                          _PD, _PD, _PD]),
+    "tdmd_eam_tables_create_values": (C.c_int, [C.POINTER(_VP), _I32, _I32, _I32, _D, _I32,
+                                                _D, _D, _PD, _PD, _PD]),
     "tdmd_eam_tables_destroy": (C.c_int, [_VP]),
     "tdmd_eam_tables_info": (C.c_int, [_VP, _PI32, _PD]),
     "tdmd_eam_compute": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _I32, C.POINTER(Box),
@@ -149,6 +153,8 @@ _PROTOS = {
                                           C.POINTER(MorseParams), C.POINTER(SchedConfig)]),
     "tdmd_sched_create_eam": (C.c_int, [C.POINTER(_VP), C.POINTER(Box), _I32, _I32, _I32, _D,
                                         _I32, _D, _D, _PD, _PD, _PD, C.POINTER(SchedConfig)]),
+    "tdmd_sched_create_eam_tables": (C.c_int, [C.POINTER(_VP), C.POINTER(Box), _I32, _VP,
+                                               C.POINTER(SchedConfig)]),
     "tdmd_sched_destroy": (C.c_int, [_VP]),
     "tdmd_sched_upload": (C.c_int, [_VP, _PD, _PD, _PD, _PI32, _PI32, _PD, _I32]),
     "tdmd_sched_run_until": (C.c_int, [_VP, _I32]),

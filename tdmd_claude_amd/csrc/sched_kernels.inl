@@ -523,13 +523,22 @@ __global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, Per
 // ELL list over the sorted atoms. Inclusion exactly as the CPU list
 // (neighbor_list.cpp:58-62): d = x_j - x_i in FP64, single minimum image,
 // unfused r2 < r_list^2. kCount: only count (capacity sizing).
+//
+// Each row is partitioned: neighbours inside the interaction cutoff at build
+// time first, the skin shell (rc <= r < r_list) after them. Atoms move less
+// than half the skin between builds, so in the force passes the in-cutoff
+// test is true for the leading slots and false for the trailing ones in
+// nearly every lane of a wave, and whole waves skip the pair work of the
+// shell. Near entries are written from the front and shell entries from the
+// back of the row; the shell block is then moved down behind the near block.
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock)
-k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, int32_t* nbr, int32_t* cnt,
-            int32_t* maxcnt) {
+k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, double near_sq, int32_t* nbr,
+            int32_t* cnt, int32_t* maxcnt) {
   if (!kCount && !a.flags[0]) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  int count = 0;
+  int count = 0, nn = 0, nf = 0;
+  const int cap = a.cap;
   if (i < a.natoms) {
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
     int c[3];
@@ -550,8 +559,14 @@ k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, int32_t* nbr, int32_t* cnt
             const double ddx = min_image(a.px[j] - xi, a.mp.L[0], a.mp.hL[0], a.mp.periodic[0]);
             const double ddy = min_image(a.py[j] - yi, a.mp.L[1], a.mp.hL[1], a.mp.periodic[1]);
             const double ddz = min_image(a.pz[j] - zi, a.mp.L[2], a.mp.hL[2], a.mp.periodic[2]);
-            if (r2_unfused(ddx, ddy, ddz) < r_list_sq) {
-              if (!kCount && count < a.cap) nbr[ell(i, count, a.cap)] = j;
+            const double r2 = r2_unfused(ddx, ddy, ddz);
+            if (r2 < r_list_sq) {
+              if (!kCount && nn + nf < cap) {
+                if (r2 < near_sq)
+                  nbr[ell(i, nn++, cap)] = j;
+                else
+                  nbr[ell(i, cap - 1 - nf++, cap)] = j;
+              }
               ++count;
             }
           }
@@ -559,18 +574,23 @@ k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, int32_t* nbr, int32_t* cnt
       }
     }
     if (!kCount) {
-      const int n = count < a.cap ? count : a.cap;
+      // Shell entries sit at [cap - nf, cap); the gap [nn, cap - nf) has
+      // room for the m that must move (the rest already lie in [nn, nn+nf)).
+      const int gap = cap - nn - nf;
+      const int m = nf < gap ? nf : gap;
+      for (int q = 0; q < m; ++q) nbr[ell(i, nn + q, cap)] = nbr[ell(i, cap - 1 - q, cap)];
+      const int n = nn + nf;  // == min(count, cap)
       cnt[i] = n;
-      // Pad the row to a multiple of 4 with the atom itself, so unrolled
-      // readers can load four slots at a time (cap is a multiple of 4).
-      for (int k = n; k < ((n + 3) & ~3); ++k) nbr[ell(i, k, a.cap)] = i;
+      // Pad the row to a multiple of 8 with the atom itself, so unrolled
+      // readers can load whole groups of slots (cap is a multiple of 8).
+      for (int k = n; k < ((n + 7) & ~7); ++k) nbr[ell(i, k, cap)] = i;
     }
   }
   const int m = wave_max_i(count);
   if ((threadIdx.x & 63) == 0 && m > 0) {
     if (kCount) {
       atomicMax(maxcnt, m);
-    } else if (m > a.cap) {
+    } else if (m > cap) {
       atomicMax(&a.flags[1], m);
     }
   }

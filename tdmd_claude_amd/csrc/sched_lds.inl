@@ -1,19 +1,23 @@
 // sched_lds.inl -- single-element EAM passes with the spline tables staged
-// in LDS and the neighbour loop unrolled so several gathers are in flight.
+// in LDS and a software-pipelined neighbour loop.
 //
 // One 1024-thread workgroup per tile of 1024 (spatially sorted) atoms. At
 // entry the workgroup copies a window of spline intervals -- the last `nwin`
 // intervals of the r grid, i.e. [r_lo, rc) -- from the coefficient table into
 // LDS; a pair closer than r_lo reads the interval from global memory instead
 // (same coefficients, same arithmetic). The window is sized to the LDS
-// budget: for the 5000-point tables of the benchmark it starts near 1 A
-// (mixed: in fact the whole table fits) and 1.8 A (FP64), below any
+// budget: for the 5000-point tables of the benchmark the whole table fits
+// in mixed precision and the FP64 window starts near 1.8 A, below any
 // distance a solid at a few hundred kelvin reaches. kWin = false is the
 // specialisation for a window that holds every interval (no global path).
 //
-// Neighbour rows are padded to a multiple of 4 (with the atom itself), so
-// the four index loads of an unrolled group need no predicate; the padded
-// slots are excluded by k + u < n.
+// Neighbour loop: slots are visited in groups of kU. The indices of the next
+// group are loaded while the current group's records are gathered and
+// evaluated, so an index load (a coalesced 256-byte line per slot and wave,
+// from HBM) overlaps a whole group of pair work instead of heading a
+// dependent load chain. Rows are padded to a multiple of 8 slots with the
+// atom itself, so group loads need no predicate; padded slots are excluded
+// by k + u < n.
 //
 // Force window entry: {phi b, phi c, phi d, rho b} + {rho c, rho d}
 // (+ phi a when the energy is wanted); density window entry: rho {a,b,c,d}.
@@ -24,7 +28,7 @@ namespace tdmd_hip {
 namespace sched {
 
 constexpr int kBigBlock = 1024;
-constexpr int kUnroll = 4;
+constexpr int kRowPad = 8;  // rows padded to a multiple of this (cap too)
 
 struct LdsWindow {
   int k_lo;  // first interval held in LDS
@@ -58,7 +62,28 @@ __device__ __forceinline__ int big_block_index(int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
-template <class R, class Mir, bool kEnergy, bool kWin>
+// Visit the n listed neighbours of atom i: pair(mj, in_row) for every slot
+// of the padded row; in_row is false for the padding.
+template <class T, int kU, class Pair>
+__device__ __forceinline__ void for_neighbours(const StepArgs& a, int i, int n,
+                                               const T* __restrict__ rec, Pair&& pair) {
+  int jn[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) jn[u] = n > 0 ? a.nbr[ell(i, u, a.cap)] : i;
+  for (int k = 0; k < n; k += kU) {
+    T mj[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) mj[u] = rec[jn[u]];
+    if (k + kU < n) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) jn[u] = a.nbr[ell(i, k + kU + u, a.cap)];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) pair(mj[u], k + u < n);
+  }
+}
+
+template <class R, class Mir, bool kEnergy, bool kWin, int kU>
 __global__ void __launch_bounds__(kBigBlock)
 k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
                   const typename Quad<R>::type* __restrict__ coeff, double rc_sq, LdsWindow w,
@@ -78,27 +103,18 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     const T mi = M[i];
     const int n = a.cnt[i];
     double rho = 0.0;
-    for (int k = 0; k < n; k += kUnroll) {
-      int j[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) j[u] = a.nbr[ell(i, k + u, a.cap)];
-      T mj[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) mj[u] = M[j[u]];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        double dx, dy, dz;
-        const double r2 = Mir::delta(mi, mj[u], a.mp, &dx, &dy, &dz);
-        if (k + u < n && r2 < rc_sq) {
-          const PairR<R> pr = pair_r<R>(r2);
-          R t;
-          const int idx = spline_locate<R>(mrho, pr.r, &t);
-          const int q = idx - w.k_lo;
-          const Q c = (!kWin || q >= 0) ? tab[q] : coeff[mrho.offset + idx];
-          rho += static_cast<double>(cubic_value<R>(c.x, c.y, c.z, c.w, t));
-        }
+    for_neighbours<T, kU>(a, i, n, M, [&](const T& mj, bool in_row) {
+      double dx, dy, dz;
+      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      if (in_row && r2 < rc_sq) {
+        const PairR<R> pr = pair_r<R>(r2);
+        R t;
+        const int idx = spline_locate<R>(mrho, pr.r, &t);
+        const int q = idx - w.k_lo;
+        const Q c = (!kWin || q >= 0) ? tab[q] : coeff[mrho.offset + idx];
+        rho += static_cast<double>(cubic_value<R>(c.x, c.y, c.z, c.w, t));
       }
-    }
+    });
     const SplineVal<R> F = spline_both<R>(mF, coeff, rho);
     static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
     e = static_cast<double>(F.f);
@@ -106,7 +122,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   if (kEnergy) emit_partial<kBigBlock>(e, a.partials, blk);
 }
 
-template <class R, class Mir, int kEpi, bool kEnergy, bool kWin>
+template <class R, class Mir, int kEpi, bool kEnergy, bool kWin, int kU>
 __global__ void __launch_bounds__(kBigBlock)
 k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
                 const typename Quad<R>::type* __restrict__ coeff, double rc_sq, LdsWindow w,
@@ -144,44 +160,35 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     const R fpi = Mir::template w_real<R>(mi);
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-    for (int k = 0; k < n; k += kUnroll) {
-      int j[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) j[u] = a.nbr[ell(i, k + u, a.cap)];
-      T mj[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) mj[u] = M2[j[u]];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        double dx, dy, dz;
-        const double r2 = Mir::delta(mi, mj[u], a.mp, &dx, &dy, &dz);
-        if (k + u < n && r2 < rc_sq) {
-          const PairR<R> pr = pair_r<R>(r2);
-          R t;
-          const int idx = spline_locate<R>(mphi, pr.r, &t);
-          const int q = idx - w.k_lo;
-          R dphi, drho;
-          if (!kWin || q >= 0) {
-            const Q x = tA[q];
-            const V2 y = tB[q];
-            dphi = cubic_deriv<R>(x.x, x.y, x.z, t);
-            drho = cubic_deriv<R>(x.w, y.x, y.y, t);
-            if (kEnergy) pe += 0.5 * static_cast<double>(cubic_value<R>(tE[q], x.x, x.y, x.z, t));
-          } else {
-            const Q cp = coeff[mphi.offset + idx];
-            const Q cr = coeff[mrho.offset + idx];
-            dphi = cubic_deriv<R>(cp.y, cp.z, cp.w, t);
-            drho = cubic_deriv<R>(cr.y, cr.z, cr.w, t);
-            if (kEnergy) pe += 0.5 * static_cast<double>(cubic_value<R>(cp.x, cp.y, cp.z, cp.w, t));
-          }
-          const R fpj = Mir::template w_real<R>(mj[u]);
-          const double fpair = static_cast<double>(-(dphi + fpi * drho + fpj * drho) * pr.rinv);
-          fx = fma(fpair, dx, fx);
-          fy = fma(fpair, dy, fy);
-          fz = fma(fpair, dz, fz);
+    for_neighbours<T, kU>(a, i, n, M2, [&](const T& mj, bool in_row) {
+      double dx, dy, dz;
+      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      if (in_row && r2 < rc_sq) {
+        const PairR<R> pr = pair_r<R>(r2);
+        R t;
+        const int idx = spline_locate<R>(mphi, pr.r, &t);
+        const int q = idx - w.k_lo;
+        R dphi, drho;
+        if (!kWin || q >= 0) {
+          const Q x = tA[q];
+          const V2 y = tB[q];
+          dphi = cubic_deriv<R>(x.x, x.y, x.z, t);
+          drho = cubic_deriv<R>(x.w, y.x, y.y, t);
+          if (kEnergy) pe += 0.5 * static_cast<double>(cubic_value<R>(tE[q], x.x, x.y, x.z, t));
+        } else {
+          const Q cp = coeff[mphi.offset + idx];
+          const Q cr = coeff[mrho.offset + idx];
+          dphi = cubic_deriv<R>(cp.y, cp.z, cp.w, t);
+          drho = cubic_deriv<R>(cr.y, cr.z, cr.w, t);
+          if (kEnergy) pe += 0.5 * static_cast<double>(cubic_value<R>(cp.x, cp.y, cp.z, cp.w, t));
         }
+        const R fpj = Mir::template w_real<R>(mj);
+        const double fpair = static_cast<double>(-(dphi + fpi * drho + fpj * drho) * pr.rinv);
+        fx = fma(fpair, dx, fx);
+        fy = fma(fpair, dy, fy);
+        fz = fma(fpair, dz, fz);
       }
-    }
+    });
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
     red = kEnergy ? pe : ke;
   }

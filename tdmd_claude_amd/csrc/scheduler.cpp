@@ -127,8 +127,11 @@ struct Engine final : EngineBase {
   }
 
   // Single-element EAM runs the LDS-table kernels (sched_lds.inl); the
-  // windows are sized once from the device's LDS limit.
+  // windows are sized once from the LDS budget (the device's per-workgroup
+  // limit unless the configuration asks for less, e.g. to fit two
+  // workgroups per CU).
   bool lds = false;
+  int unroll = 4;
   LdsWindow w_dens[2]{}, w_force[2]{};  // [kEnergy]
   size_t b_dens[2]{}, b_force[2]{};
 
@@ -147,16 +150,16 @@ struct Engine final : EngineBase {
     return TDMD_OK;
   }
 
-  template <bool kWin>
+  template <bool kWin, int kU>
   int allow_all() {
     int rc = 0;
-    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin>, b_dens[0])) ||
-        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin>, b_dens[1])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin>, b_force[1])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin>, b_force[0])))
+    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin, kU>, b_dens[0])) ||
+        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin, kU>, b_dens[1])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin, kU>, b_force[1])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin, kU>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU>, b_force[0])) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU>, b_force[0])))
       return rc;
     return TDMD_OK;
   }
@@ -174,48 +177,61 @@ struct Engine final : EngineBase {
     TDMD_HIP_TRY(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     // The dynamic window plus the kernels' static LDS (block reduction)
     // must fit the per-workgroup limit; keep 1 KiB for the static part.
-    const size_t budget = static_cast<size_t>(std::min(lmax, 160 * 1024) - 1024);
+    int limit = std::min(lmax, 160 * 1024) - 1024;
+    if (S->cfg.lds_budget > 0) limit = std::min(limit, S->cfg.lds_budget);
+    if (limit < 4096) return fail(TDMD_E_INVALID, "scheduler: lds_budget below 4096 bytes");
+    const size_t budget = static_cast<size_t>(limit);
     const int nint = m[1].n - 1;
     using V2 = typename Vec2<R>::type;
     for (int e = 0; e < 2; ++e) {
       w_dens[e] = window(nint, sizeof(Q), budget, &b_dens[e]);
       w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0), budget, &b_force[e]);
     }
+    unroll = S->cfg.pair_unroll == 0 ? 8 : S->cfg.pair_unroll;  // 8: fastest measured
     int rc = 0;
-    if ((rc = allow_all<false>()) || (rc = allow_all<true>())) return rc;
+    if ((rc = allow_all<false, 4>()) || (rc = allow_all<true, 4>()) ||
+        (rc = allow_all<false, 8>()) || (rc = allow_all<true, 8>()))
+      return rc;
     lds = true;
     S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
     return TDMD_OK;
   }
 
+  template <bool kEnergy, bool kWin, int kU>
+  void density_lds(hipStream_t s) {
+    const int nb = S->nb_force;
+    k_eam_density_lds<R, Mir, kEnergy, kWin, kU><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
+        S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
+  }
   template <bool kEnergy>
   void density(hipStream_t s) {
-    if (lds) {
-      const int nb = S->nb_force;
-      const LdsWindow& w = w_dens[kEnergy];
-      if (w.k_lo > 0)
-        k_eam_density_lds<R, Mir, kEnergy, true><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
-            S->a, meta(), coeff(), rc_sq(), w, nb);
-      else
-        k_eam_density_lds<R, Mir, kEnergy, false><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
-            S->a, meta(), coeff(), rc_sq(), w, nb);
-    } else {
+    if (!lds) {
       k_eam_density<R, Mir, kEnergy><<<S->nblocks, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+      return;
     }
+    const bool win = w_dens[kEnergy].k_lo > 0;
+    if (unroll == 8)
+      win ? density_lds<kEnergy, true, 8>(s) : density_lds<kEnergy, false, 8>(s);
+    else
+      win ? density_lds<kEnergy, true, 4>(s) : density_lds<kEnergy, false, 4>(s);
+  }
+
+  template <int kEpi, bool kEnergy, bool kWin, int kU>
+  void force_lds(hipStream_t s) {
+    const int nb = S->nb_force;
+    k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU><<<nb, kBigBlock, b_force[kEnergy], s>>>(
+        S->a, meta(), coeff(), rc_sq(), w_force[kEnergy], nb);
   }
 
   template <int kEpi, bool kEnergy>
   void force(hipStream_t s) {
     const int nb = S->nblocks;
     if (S->kind == 1) {
-      if (lds && w_force[kEnergy].k_lo > 0)
-        k_eam_force_lds<R, Mir, kEpi, kEnergy, true>
-            <<<S->nb_force, kBigBlock, b_force[kEnergy], s>>>(S->a, meta(), coeff(), rc_sq(),
-                                                               w_force[kEnergy], S->nb_force);
+      const bool win = w_force[kEnergy].k_lo > 0;
+      if (lds && unroll == 8)
+        win ? force_lds<kEpi, kEnergy, true, 8>(s) : force_lds<kEpi, kEnergy, false, 8>(s);
       else if (lds)
-        k_eam_force_lds<R, Mir, kEpi, kEnergy, false>
-            <<<S->nb_force, kBigBlock, b_force[kEnergy], s>>>(S->a, meta(), coeff(), rc_sq(),
-                                                               w_force[kEnergy], S->nb_force);
+        win ? force_lds<kEpi, kEnergy, true, 4>(s) : force_lds<kEpi, kEnergy, false, 4>(s);
       else if (S->tables->ntypes > 1)
         k_eam_force<R, Mir, kEpi, kEnergy, true><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
       else
@@ -242,6 +258,7 @@ struct Engine final : EngineBase {
   int fill_pass(hipStream_t s) override {
     const int gb = grid_for(S->natoms);
     k_nlist_ell<false><<<gb, kBlock, 0, s>>>(S->a, S->g, S->r_list * S->r_list,
+                                             S->cutoff * S->cutoff,
                                              const_cast<int32_t*>(S->a.nbr),
                                              const_cast<int32_t*>(S->a.cnt), nullptr);
     k_rebuild_done<<<gb, kBlock, 0, s>>>(S->a.flags, S->g.done, gb, S->d_rebuilds);
@@ -251,7 +268,8 @@ struct Engine final : EngineBase {
   }
   int count_pass(hipStream_t s, int32_t* d_max) override {
     k_nlist_ell<true><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a, S->g, S->r_list * S->r_list,
-                                                             nullptr, nullptr, d_max);
+                                                             S->cutoff * S->cutoff, nullptr,
+                                                             nullptr, d_max);
     TDMD_LAUNCH_CHECK();
     return TDMD_OK;
   }
@@ -435,6 +453,9 @@ int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched
     return fail(TDMD_E_INVALID, "scheduler: bad precision");
   if (cfg.t_target > 0 && (cfg.nhc_length < 1 || cfg.nhc_length > 8 || !(cfg.t_period > 0)))
     return fail(TDMD_E_INVALID, "scheduler: NHC chain length must be 1..8, t_period > 0");
+  if (cfg.pair_unroll != 0 && cfg.pair_unroll != 4 && cfg.pair_unroll != 8)
+    return fail(TDMD_E_INVALID, "scheduler: pair_unroll must be 0 (default), 4 or 8");
+  if (cfg.lds_budget < 0) return fail(TDMD_E_INVALID, "scheduler: lds_budget must be >= 0");
   S->box = box;
   S->natoms = natoms;
   S->cfg = cfg;
@@ -530,7 +551,7 @@ int first_build(tdmd_sched* S) {
   TDMD_HIP_TRY(hipStreamSynchronize(s));
   int cap = S->cfg.max_neighbors > 0 ? S->cfg.max_neighbors
                                      : round_up(static_cast<int>(std::ceil(maxc * 1.25)) + 8, 4);
-  cap = round_up(std::max(cap, 4), 4);  // rows are padded to multiples of 4
+  cap = round_up(std::max(cap, kRowPad), kRowPad);  // rows are padded to kRowPad slots
   if (S->a.nbr == nullptr || cap > S->a.cap) {
     if (S->a.nbr) TDMD_HIP_TRY(hipFree(const_cast<int32_t*>(S->a.nbr)));
     void* p = nullptr;
@@ -716,6 +737,30 @@ int tdmd_sched_create_eam(tdmd_sched** out, const tdmd_box* box, int32_t natoms,
     return rc;
   }
   S->cutoff = cutoff;  // fast_pipeline_scheduler.cu:35 (cutoff_ = eam.cutoff())
+  rc = sched_setup(S, *box, natoms, *cfg);
+  if (rc) {
+    sched_free(S);
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return TDMD_OK;
+}
+
+int tdmd_sched_create_eam_tables(tdmd_sched** out, const tdmd_box* box, int32_t natoms,
+                                 const tdmd_eam_tables* tables, const tdmd_sched_config* cfg) {
+  if (!out || !box || !cfg || !tables)
+    return fail(TDMD_E_INVALID, "sched_create_eam_tables: null");
+  if (tables->ntypes < 1 || tables->ntypes > sched::kMaxTypes)
+    return fail(TDMD_E_INVALID, "sched_create_eam_tables: 1..8 element types");
+  auto* S = new tdmd_sched();
+  S->kind = 1;
+  int rc = eam_tables_clone(tables, cfg->precision, &S->tables);
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  S->cutoff = tables->cutoff;
   rc = sched_setup(S, *box, natoms, *cfg);
   if (rc) {
     sched_free(S);

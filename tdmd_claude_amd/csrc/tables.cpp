@@ -169,14 +169,43 @@ int eam_tables_create(tdmd_eam_tables** out, int precision, const SetflData& s) 
       const int pidx = phi_index(ii, jj, nt);
       const size_t o2 = static_cast<size_t>(t->offsets[static_cast<size_t>(2 * nt + pidx)]);
       const double* rp = s.rphi.data() + static_cast<size_t>(p_read) * s.nr;
-      vals.assign(static_cast<size_t>(s.nr), 0.0);
-      for (int q = 1; q < s.nr; ++q) {
-        const double r = static_cast<double>(q) * s.dr;
-        vals[static_cast<size_t>(q)] = rp[q] / r;
+      if (s.phi_values) {
+        vals.assign(rp, rp + s.nr);
+      } else {
+        vals.assign(static_cast<size_t>(s.nr), 0.0);
+        for (int q = 1; q < s.nr; ++q) {
+          const double r = static_cast<double>(q) * s.dr;
+          vals[static_cast<size_t>(q)] = rp[q] / r;
+        }
       }
       build_fd_hermite(vals.data(), s.nr, s.dr, &t->a[o2], &t->b[o2], &t->c[o2], &t->d[o2]);
     }
   }
+  const int rc = precision == TDMD_MIXED ? upload_quads<float>(t) : upload_quads<double>(t);
+  if (rc) {
+    eam_tables_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return TDMD_OK;
+}
+
+int eam_tables_clone(const tdmd_eam_tables* src, int precision, tdmd_eam_tables** out) {
+  if (precision != TDMD_FP64 && precision != TDMD_MIXED)
+    return fail(TDMD_E_INVALID, "eam tables: bad precision");
+  auto* t = new tdmd_eam_tables();
+  t->precision = precision;
+  t->ntypes = src->ntypes;
+  t->nrho = src->nrho;
+  t->nr = src->nr;
+  t->drho = src->drho;
+  t->dr = src->dr;
+  t->cutoff = src->cutoff;
+  t->a = src->a;
+  t->b = src->b;
+  t->c = src->c;
+  t->d = src->d;
+  t->offsets = src->offsets;
   const int rc = precision == TDMD_MIXED ? upload_quads<float>(t) : upload_quads<double>(t);
   if (rc) {
     eam_tables_destroy(t);
@@ -242,6 +271,27 @@ int tdmd_eam_tables_create(tdmd_eam_tables** out, int32_t precision, int32_t nty
   s.F.assign(F, F + static_cast<size_t>(ntypes) * nrho);
   s.rho.assign(rho, rho + static_cast<size_t>(ntypes) * nr);
   s.rphi.assign(rphi, rphi + static_cast<size_t>(np) * nr);
+  return eam_tables_create(out, precision, s);
+}
+
+int tdmd_eam_tables_create_values(tdmd_eam_tables** out, int32_t precision, int32_t ntypes,
+                                  int32_t nrho, double drho, int32_t nr, double dr,
+                                  double cutoff, const double* F, const double* rho,
+                                  const double* phi) {
+  if (!out || !F || !rho || !phi || ntypes <= 0 || nrho < 2 || nr < 2)
+    return fail(TDMD_E_INVALID, "eam_tables_create_values: bad arguments");
+  SetflData s;
+  s.ntypes = ntypes;
+  s.nrho = nrho;
+  s.nr = nr;
+  s.drho = drho;
+  s.dr = dr;
+  s.cutoff = cutoff;
+  s.phi_values = true;
+  const int np = ntypes * (ntypes + 1) / 2;
+  s.F.assign(F, F + static_cast<size_t>(ntypes) * nrho);
+  s.rho.assign(rho, rho + static_cast<size_t>(ntypes) * nr);
+  s.rphi.assign(phi, phi + static_cast<size_t>(np) * nr);
   return eam_tables_create(out, precision, s);
 }
 

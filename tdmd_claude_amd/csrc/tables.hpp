@@ -40,6 +40,7 @@ struct SetflData {
   int ntypes = 0, nrho = 0, nr = 0;
   double drho = 0, dr = 0, cutoff = 0;
   std::vector<double> F, rho, rphi;  // rphi blocks in i<=j row order
+  bool phi_values = false;           // rphi already holds phi = (r phi) / r
 };
 
 int read_setfl(const std::string& path, SetflData* out);
@@ -74,5 +75,7 @@ struct tdmd_eam_tables {
 
 namespace tdmd_hip {
 int eam_tables_create(tdmd_eam_tables** out, int precision, const SetflData& s);
+// Same coefficients, uploaded again in `precision` (host FP64 copies).
+int eam_tables_clone(const tdmd_eam_tables* src, int precision, tdmd_eam_tables** out);
 void eam_tables_destroy(tdmd_eam_tables* t);
 }  // namespace tdmd_hip

@@ -39,6 +39,8 @@ class FastPipelineConfig:
     use_graph: bool = True
     mirror_fp64: bool = False
     max_neighbors: int = 0
+    pair_unroll: int = 0  # neighbour slots per pipelined group (0 = default)
+    lds_budget: int = 0  # LDS bytes per workgroup for spline windows (0 = device limit)
 
     def to_c(self) -> capi.SchedConfig:
         c = capi.SchedConfig()
@@ -55,6 +57,8 @@ class FastPipelineConfig:
         c.use_graph = int(self.use_graph)
         c.mirror_fp64 = int(self.mirror_fp64)
         c.max_neighbors = self.max_neighbors
+        c.pair_unroll = self.pair_unroll
+        c.lds_budget = self.lds_budget
         return c
 
 
