@@ -182,7 +182,9 @@ __global__ void k_nhc_pre(double* nhc, NhcParams p) {
 }
 
 // After the VV step: KE from the per-block partials of the kick-2 epilogue,
-// half-step, pending scale = s2.
+// half-step, pending scale = s2. kPre: the next step's k_nhc_pre follows in
+// the same thread (same operations, one launch less per step).
+template <bool kPre>
 __global__ void k_nhc_post(double* nhc, const double* partials, int n, NhcParams p) {
   __shared__ double s[256];
   double t = 0.0;
@@ -199,6 +201,13 @@ __global__ void k_nhc_post(double* nhc, const double* partials, int n, NhcParams
                                     p.n_dof, ke);
     nhc[25] = ke;
     nhc[24] = s2;
+    if (kPre) {
+      const double ke1 = nhc[24] * nhc[24] * nhc[25];
+      const double s1 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
+                                      p.n_dof, ke1);
+      nhc[26] = s1 * nhc[24];
+      nhc[24] = 1.0;
+    }
   }
 }
 
@@ -227,7 +236,45 @@ __global__ void k_nhc_commit(double* nhc) {
 }
 
 // ---------------------------------------------------------------------------
-// EAM passes
+// Neighbour loop shared by the force passes.
+//
+// Slots are visited in groups of kU. The indices of the next group are
+// loaded while the current group's records are gathered and evaluated, so an
+// index load (a coalesced 256-byte line per slot and wave, from HBM) overlaps
+// a whole group of pair work instead of heading a dependent load chain. Rows
+// are padded to a multiple of kRowPad slots with the atom itself, so group
+// loads need no predicate; padded slots arrive with in_row = false.
+// ---------------------------------------------------------------------------
+
+constexpr int kRowPad = 8;  // rows padded to a multiple of this (cap too)
+constexpr int kPairUnroll = 8;
+
+template <class T, int kU, class Pair>
+__device__ __forceinline__ void for_neighbours(const StepArgs& a, int i, int n,
+                                               const T* __restrict__ rec, Pair&& pair) {
+  static_assert(kRowPad % kU == 0, "groups must tile the padded row");
+  int jn[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) jn[u] = n > 0 ? a.nbr[ell(i, u, a.cap)] : i;
+  for (int k = 0; k < n; k += kU) {
+    int j[kU];
+    T mj[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      j[u] = jn[u];
+      mj[u] = rec[j[u]];
+    }
+    if (k + kU < n) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) jn[u] = a.nbr[ell(i, k + kU + u, a.cap)];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) pair(j[u], mj[u], k + u < n);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// EAM passes (global tables; any number of elements)
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void load_meta(SplineMeta* sm, const SplineMeta* meta, int ns) {
@@ -251,16 +298,14 @@ k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
     const int ti = Mir::type_of(mi);
     const int n = a.cnt[i];
     double rho = 0.0;
-    for (int k = 0; k < n; ++k) {
-      const int j = a.nbr[ell(i, k, a.cap)];
-      const T mj = M[j];
+    for_neighbours<T, kPairUnroll>(a, i, n, M, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
-      if (r2 < rc_sq) {
+      if (in_row && r2 < rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         rho += static_cast<double>(spline_value<R>(sm[a.ntypes + Mir::type_of(mj)], coeff, pr.r));
       }
-    }
+    });
     const SplineVal<R> F = spline_both<R>(sm[ti], coeff, rho);
     static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
     e = static_cast<double>(F.f);
@@ -286,12 +331,10 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
     const int ti = kMulti ? a.types[i] - 1 : 0;
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-    for (int k = 0; k < n; ++k) {
-      const int j = a.nbr[ell(i, k, a.cap)];
-      const T mj = M2[j];
+    for_neighbours<T, kPairUnroll>(a, i, n, M2, [&](int j, const T& mj, bool in_row) {
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
-      if (r2 < rc_sq) {
+      if (in_row && r2 < rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         const int tj = kMulti ? a.types[j] - 1 : 0;
         const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
@@ -311,7 +354,7 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
         fy = fma(fpair, dy, fy);
         fz = fma(fpair, dz, fz);
       }
-    }
+    });
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
     red = kEnergy ? pe : ke;
   }
@@ -335,12 +378,10 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
     const T mi = M[i];
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-    for (int k = 0; k < n; ++k) {
-      const int j = a.nbr[ell(i, k, a.cap)];
-      const T mj = M[j];
+    for_neighbours<T, kPairUnroll>(a, i, n, M, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
-      if (r2 < p.rc_sq) {
+      if (in_row && r2 < p.rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         R e;
         const double fpair = static_cast<double>(morse_fpair<R>(p, pr, &e));
@@ -349,7 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
         fy = fma(fpair, dy, fy);
         fz = fma(fpair, dz, fz);
       }
-    }
+    });
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
     red = kEnergy ? pe : ke;
   }

@@ -11,13 +11,8 @@
 // distance a solid at a few hundred kelvin reaches. kWin = false is the
 // specialisation for a window that holds every interval (no global path).
 //
-// Neighbour loop: slots are visited in groups of kU. The indices of the next
-// group are loaded while the current group's records are gathered and
-// evaluated, so an index load (a coalesced 256-byte line per slot and wave,
-// from HBM) overlaps a whole group of pair work instead of heading a
-// dependent load chain. Rows are padded to a multiple of 8 slots with the
-// atom itself, so group loads need no predicate; padded slots are excluded
-// by k + u < n.
+// The neighbour loop is for_neighbours (sched_kernels.inl), software
+// pipelined in groups of kU slots.
 //
 // Force window entry: {phi b, phi c, phi d, rho b} + {rho c, rho d}
 // (+ phi a when the energy is wanted); density window entry: rho {a,b,c,d}.
@@ -28,7 +23,6 @@ namespace tdmd_hip {
 namespace sched {
 
 constexpr int kBigBlock = 1024;
-constexpr int kRowPad = 8;  // rows padded to a multiple of this (cap too)
 
 struct LdsWindow {
   int k_lo;  // first interval held in LDS
@@ -62,27 +56,6 @@ __device__ __forceinline__ int big_block_index(int nblocks) {
   return (b & 7) * per + (b >> 3);
 }
 
-// Visit the n listed neighbours of atom i: pair(mj, in_row) for every slot
-// of the padded row; in_row is false for the padding.
-template <class T, int kU, class Pair>
-__device__ __forceinline__ void for_neighbours(const StepArgs& a, int i, int n,
-                                               const T* __restrict__ rec, Pair&& pair) {
-  int jn[kU];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) jn[u] = n > 0 ? a.nbr[ell(i, u, a.cap)] : i;
-  for (int k = 0; k < n; k += kU) {
-    T mj[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) mj[u] = rec[jn[u]];
-    if (k + kU < n) {
-#pragma unroll
-      for (int u = 0; u < kU; ++u) jn[u] = a.nbr[ell(i, k + kU + u, a.cap)];
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) pair(mj[u], k + u < n);
-  }
-}
-
 template <class R, class Mir, bool kEnergy, bool kWin, int kU>
 __global__ void __launch_bounds__(kBigBlock)
 k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
@@ -103,7 +76,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     const T mi = M[i];
     const int n = a.cnt[i];
     double rho = 0.0;
-    for_neighbours<T, kU>(a, i, n, M, [&](const T& mj, bool in_row) {
+    for_neighbours<T, kU>(a, i, n, M, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
@@ -160,7 +133,7 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     const R fpi = Mir::template w_real<R>(mi);
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-    for_neighbours<T, kU>(a, i, n, M2, [&](const T& mj, bool in_row) {
+    for_neighbours<T, kU>(a, i, n, M2, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {

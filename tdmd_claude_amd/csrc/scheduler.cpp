@@ -51,6 +51,7 @@ enum Op : char {
   OP_F_HALF = 'h',      // force + closing half kick
   OP_F_KICK2 = '2',     // force + half kick + KE partials (NVT)
   OP_NVT_POST = 'P',    // chain half-step after the step
+  OP_NVT_POST_PRE = 'N',// chain half-step after the step + the next step's
   OP_NVT_FINAL = 'Z'    // apply pending chain scale, commit
 };
 
@@ -311,7 +312,10 @@ struct Engine final : EngineBase {
         k_nhc_pre<<<1, 64, 0, s>>>(S->a.nhc, S->nhcp);
         break;
       case OP_NVT_POST:
-        k_nhc_post<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
+        k_nhc_post<false><<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
+        break;
+      case OP_NVT_POST_PRE:
+        k_nhc_post<true><<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
         break;
       case OP_NVT_FINAL:
         k_scale_v<<<gb, kBlock, 0, s>>>(S->a);
@@ -643,7 +647,7 @@ int run_window(tdmd_sched* S, int n) {
   for (int k = 1; k <= n; ++k) {
     std::string st;
     if (nvt) {
-      st += static_cast<char>(OP_NVT_PRE);
+      if (k == 1) st += static_cast<char>(OP_NVT_PRE);  // later steps: fused into 'N'
       st += static_cast<char>(OP_KD_NVT);
     } else if (k == 1) {
       st += static_cast<char>(OP_BOOT);
@@ -660,7 +664,7 @@ int run_window(tdmd_sched* S, int n) {
     if (eam) st += static_cast<char>(OP_DENS);
     if (nvt) {
       st += static_cast<char>(OP_F_KICK2);
-      st += static_cast<char>(OP_NVT_POST);
+      st += static_cast<char>(k < n ? OP_NVT_POST_PRE : OP_NVT_POST);
     } else if (k == n) {
       st += static_cast<char>(OP_F_HALF);
     } else {
