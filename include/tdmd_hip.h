@@ -323,6 +323,41 @@ int tdmd_sched_time_passes(tdmd_sched* s, int32_t steps, float* ms_out);
 /* Stream the scheduler runs on (for event timing). */
 int tdmd_sched_stream(const tdmd_sched* s, void** stream);
 
+/* ------------------------------------------------------------------ */
+/* Force matching (no reference counterpart: the differentiable path  */
+/* of the fitting harness). A fixed set of nsnap configurations of    */
+/* natoms atoms in one box; L = sum |F - T|^2 over all atoms and its  */
+/* gradient by a hand-written adjoint on the same pair arithmetic as  */
+/* the force kernels. One-element potentials.                          */
+/* ------------------------------------------------------------------ */
+typedef struct tdmd_fm tdmd_fm;
+
+/* pos: host double[nsnap][natoms][3]. Builds each snapshot's full list
+ * with cutoff r_cut once (allocates). */
+int tdmd_fm_create(tdmd_fm** out, int32_t precision, const tdmd_box* box,
+                   int32_t natoms, int32_t nsnap, const double* pos,
+                   double r_cut);
+int tdmd_fm_destroy(tdmd_fm* fm);
+/* Total atoms and listed (ordered) pairs. */
+int tdmd_fm_info(const tdmd_fm* fm, int32_t* ntot, int64_t* npairs);
+/* Target forces, host double[nsnap][natoms][3]. */
+int tdmd_fm_set_targets(tdmd_fm* fm, const double* forces);
+/* Morse (D, alpha, r0) with cutoff rc <= r_cut; host outputs, any may be
+ * NULL: loss, grad[3] = dL/d(D, alpha, r0), forces_out[3 ntot]. */
+int tdmd_fm_morse(tdmd_fm* fm, const double params[3], double rc,
+                  double* loss, double grad[3], double* forces_out);
+/* Single-element EAM from knot values on the device, d_values =
+ * [F(nrho) | rho(nr) | phi(nr)] (phi = (r phi)/r, EamAlloy's form).
+ * Device outputs, any may be NULL: d_loss[1], d_grad (same layout as
+ * d_values), d_forces[3 ntot]. Synchronises `stream` before returning. */
+int tdmd_fm_eam(tdmd_fm* fm, int32_t nrho, double drho, int32_t nr,
+                double dr, double cutoff, const double* d_values,
+                double* d_loss, double* d_grad, double* d_forces,
+                void* stream);
+/* Coefficient gradient of the last tdmd_fm_eam call (host copy):
+ * F{a,b,c,d}[nrho-1], rho{a,b,c,d}[nr-1], phi{a,b,c,d}[nr-1]. */
+int tdmd_fm_eam_coef_grad(const tdmd_fm* fm, double* gcoef);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,14 @@ def lib() -> C.CDLL:
         L.orc_nhc_half_step.restype = _D
         L.orc_nhc_energy.argtypes = [C.POINTER(OrcNhc)]
         L.orc_nhc_energy.restype = _D
+        L.orc_fm_morse.argtypes = [_PD, _I64, C.POINTER(OrcBox), C.POINTER(OrcNlist), _D, _D,
+                                   _D, _D, _PD, _PD, _PD]
+        L.orc_fm_morse.restype = _D
+        L.orc_eam_from_values.argtypes = [C.POINTER(OrcEam), _I, _D, _I, _D, _D, _PD, _PD, _PD]
+        L.orc_fm_eam.argtypes = [C.POINTER(OrcEam), _PD, _I64, C.POINTER(OrcBox),
+                                 C.POINTER(OrcNlist), _PD, _PD, _PD, _PD, _PD, _PD]
+        L.orc_fm_eam.restype = _D
+        L.orc_spline_coef_grad_to_values.argtypes = [_I, _D, _PD, _PD, _PD, _PD, _PD]
         L.orc_run.argtypes = [C.POINTER(OrcRunCfg), C.POINTER(OrcBox), _PD, _PD, _PD, _PI, _PD,
                               _I64, C.POINTER(OrcRunOut)]
         _lib = L
@@ -279,6 +287,51 @@ class Eam:
             lib().orc_eam_free(C.byref(self.e))
         except Exception:
             pass
+
+
+class EamValues(Eam):
+    """Single-element EAM from knot values as EamAlloy holds them:
+    F[nrho], rho[nr], phi[nr] (phi = (r phi)/r)."""
+
+    def __init__(self, F, rho, phi, drho, dr, cutoff):
+        self.e = OrcEam()
+        self._v = [np.ascontiguousarray(x, dtype=np.float64) for x in (F, rho, phi)]
+        F, rho, phi = self._v
+        if lib().orc_eam_from_values(C.byref(self.e), len(F), drho, len(rho), dr, cutoff,
+                                     _pd(F), _pd(rho), _pd(phi)) != 0:
+            raise RuntimeError("orc_eam_from_values failed")
+
+
+def fm_morse(half: HalfList, D, alpha, r0, rc, target):
+    """Force-matching loss sum |F - T|^2 and its gradient wrt (D, alpha, r0)."""
+    n = len(half.pos)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    grad = np.zeros(3)
+    f = np.zeros((n, 3))
+    L = lib().orc_fm_morse(_pd(half.pos), n, C.byref(half.box), C.byref(half.nl), D, alpha, r0,
+                           rc, _pd(target), _pd(grad), _pd(f))
+    return L, grad, f
+
+
+def fm_eam(eam: EamValues, half: HalfList, target):
+    """Loss and gradient wrt the F, rho, phi knot values (and the cubic
+    coefficients, blocks F{a,b,c,d}, rho{a,b,c,d}, phi{a,b,c,d})."""
+    n = len(half.pos)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    nrho, nr = eam.e.nrho, eam.e.nr
+    gF, grho, gphi = np.zeros(nrho), np.zeros(nr), np.zeros(nr)
+    gcoef = np.zeros(4 * (nrho - 1) + 8 * (nr - 1))
+    f = np.zeros((n, 3))
+    L = lib().orc_fm_eam(C.byref(eam.e), _pd(half.pos), n, C.byref(half.box), C.byref(half.nl),
+                         _pd(target), _pd(gF), _pd(grho), _pd(gphi), _pd(gcoef), _pd(f))
+    return L, dict(F=gF, rho=grho, phi=gphi, coef=gcoef), f
+
+
+def spline_coef_grad_to_values(n, dr, ga, gb, gc, gd):
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (ga, gb, gc, gd)]
+    gv = np.zeros(n)
+    lib().orc_spline_coef_grad_to_values(n, dr, *[_pd(x) for x in arrs], _pd(gv))
+    return gv
 
 
 def kinetic_energy(vel, types, masses):

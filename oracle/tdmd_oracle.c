@@ -676,6 +676,219 @@ double orc_eam_forces(const orc_eam* e, const double* pos, const int* types,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Force matching (no reference counterpart)                                */
+/* ------------------------------------------------------------------------ */
+
+double orc_fm_morse(const double* pos, int64_t natoms, const orc_box* box,
+                    const orc_nlist* nl, double D, double alpha, double r0,
+                    double rc, const double* target, double grad[3],
+                    double* forces_out) {
+  /* F_i = sum_j fpair(r_ij) d_ij, d_ij = x_i - x_j; for a half pair the
+   * force enters i with +d and j with -d, so dL = 2 (g_i - g_j).d dfpair,
+   * g = F - T. fpair = -2 D alpha (1-e) e / r, e = exp(-alpha (r - r0)). */
+  const size_t n3 = 3 * (size_t)(natoms > 0 ? natoms : 1);
+  double* f = (double*)malloc(n3 * sizeof(double));
+  orc_morse_forces(pos, f, natoms, box, nl, D, alpha, r0, rc);
+  if (forces_out) memcpy(forces_out, f, 3 * (size_t)natoms * sizeof(double));
+  double L = 0.0;
+  for (size_t k = 0; k < 3 * (size_t)natoms; ++k) {
+    f[k] -= target[k];
+    L += f[k] * f[k];
+  }
+  const double rc_sq = rc * rc;
+  double bs[3];
+  box_size(box, bs);
+  grad[0] = grad[1] = grad[2] = 0.0;
+  for (int64_t i = 0; i < natoms; ++i) {
+    const double* pi = pos + 3 * i;
+    const int* nb = nl->nbrs + nl->offsets[i];
+    for (int k = 0; k < nl->counts[i]; ++k) {
+      const int j = nb[k];
+      double d[3] = {pi[0] - pos[3 * j], pi[1] - pos[3 * j + 1], pi[2] - pos[3 * j + 2]};
+      orc_minimum_image(d, bs, box->periodic);
+      const double r2 = length_sq(d);
+      if (r2 >= rc_sq) continue;
+      const double r = sqrt(r2);
+      const double e = exp(-alpha * (r - r0));
+      const double w = 2.0 * ((f[3 * i] - f[3 * j]) * d[0] + (f[3 * i + 1] - f[3 * j + 1]) * d[1] +
+                              (f[3 * i + 2] - f[3 * j + 2]) * d[2]);
+      grad[0] += w * (-2.0 * alpha * (1.0 - e) * e / r);
+      grad[1] += w * (-2.0 * D * e * ((1.0 - e) - alpha * (1.0 - 2.0 * e) * (r - r0)) / r);
+      grad[2] += w * (-2.0 * D * alpha * alpha * (1.0 - 2.0 * e) * e / r);
+    }
+  }
+  free(f);
+  return L;
+}
+
+int orc_eam_from_values(orc_eam* e, int nrho, double drho, int nr, double dr,
+                        double cutoff, const double* F, const double* rho,
+                        const double* phi) {
+  memset(e, 0, sizeof(*e));
+  if (nrho < 2 || nr < 2) return -1;
+  if (eam_alloc(e, 1) != 0) return -2;
+  e->cutoff = cutoff;
+  e->nrho = nrho;
+  e->nr = nr;
+  e->drho = drho;
+  e->dr = dr;
+  set_values(&e->embedding[0], nrho, drho, F);
+  orc_spline_build(&e->embedding[0]);
+  set_values(&e->density[0], nr, dr, rho);
+  orc_spline_build(&e->density[0]);
+  set_values(&e->phi[0], nr, dr, phi);
+  orc_spline_build(&e->phi[0]);
+  return 0;
+}
+
+/* Interval and offset of x, as orc_spline_eval. */
+static int spline_locate(const orc_spline* s, double x, double* t) {
+  int idx = (int)((x - s->rmin) / s->dr);
+  if (idx < 0) idx = 0;
+  if (idx >= s->n - 1) idx = s->n - 2;
+  *t = x - (s->rmin + (double)idx * s->dr);
+  return idx;
+}
+
+void orc_spline_coef_grad_to_values(int n, double dr, const double* ga,
+                                    const double* gb, const double* gc,
+                                    const double* gd, double* gv) {
+  /* Reverse of orc_spline_build. With S_k = (v_{k+1} - v_k)/dr:
+   *   a_k = v_k, b_k = m0_k, c_k = (3 S_k - 2 m0_k - m1_k)/dr,
+   *   d_k = (m0_k + m1_k - 2 S_k)/dr^2,
+   *   m0_0 = S_0, m0_k = (S_{k-1} + S_k)/2; m1_{n-2} = S_{n-2},
+   *   m1_k = (S_k + S_{k+1})/2. */
+  const int ni = n - 1;
+  double* gS = (double*)calloc((size_t)(ni > 0 ? ni : 1), sizeof(double));
+  for (int k = 0; k < n; ++k) gv[k] = 0.0;
+  for (int k = 0; k < ni; ++k) {
+    const double gm0 = gb[k] - 2.0 * gc[k] / dr + gd[k] / (dr * dr);
+    const double gm1 = -gc[k] / dr + gd[k] / (dr * dr);
+    gS[k] += 3.0 * gc[k] / dr - 2.0 * gd[k] / (dr * dr);
+    if (k == 0) {
+      gS[0] += gm0;
+    } else {
+      gS[k - 1] += 0.5 * gm0;
+      gS[k] += 0.5 * gm0;
+    }
+    if (k == ni - 1) {
+      gS[k] += gm1;
+    } else {
+      gS[k] += 0.5 * gm1;
+      gS[k + 1] += 0.5 * gm1;
+    }
+    gv[k] += ga[k];
+  }
+  for (int k = 0; k < ni; ++k) {
+    gv[k + 1] += gS[k] / dr;
+    gv[k] -= gS[k] / dr;
+  }
+  free(gS);
+}
+
+double orc_fm_eam(const orc_eam* e, const double* pos, int64_t natoms,
+                  const orc_box* box, const orc_nlist* nl, const double* target,
+                  double* gF, double* grho, double* gphi, double* gcoef,
+                  double* forces_out) {
+  /* Single element. With g = F - T, u = d/r, w = -2 (g_i - g_j).u per half
+   * pair and Fp = F'(rho):
+   *   dL = sum_pairs w [dphi' + (Fp_i + Fp_j) drho'] + sum_i lam_i dFp_i,
+   *   lam_i = sum_{j} w_ij rho'(r_ij),
+   *   dFp_i = dF'(rho_i)|coef + F''(rho_i) drho_i, drho_i = sum_j drho(r_ij). */
+  const size_t N = (size_t)(natoms > 0 ? natoms : 1);
+  double* f = (double*)malloc(3 * N * sizeof(double));
+  double* rho = (double*)malloc(N * sizeof(double));
+  double* fp = (double*)malloc(N * sizeof(double));
+  double* lam = (double*)calloc(N, sizeof(double));
+  int* types = (int*)malloc(N * sizeof(int));
+  for (size_t i = 0; i < N; ++i) types[i] = 1;
+  orc_eam_forces(e, pos, types, f, natoms, box, nl, rho, fp, NULL);
+  if (forces_out) memcpy(forces_out, f, 3 * (size_t)natoms * sizeof(double));
+  double L = 0.0;
+  for (size_t k = 0; k < 3 * (size_t)natoms; ++k) {
+    f[k] -= target[k];
+    L += f[k] * f[k];
+  }
+  const orc_spline* sF = &e->embedding[0];
+  const orc_spline* sR = &e->density[0];
+  const orc_spline* sP = &e->phi[0];
+  const int niF = sF->n - 1, niR = sR->n - 1;
+  /* coefficient gradients, blocks [4][n-1] */
+  double* cF = (double*)calloc(4 * (size_t)niF, sizeof(double));
+  double* cR = (double*)calloc(4 * (size_t)niR, sizeof(double));
+  double* cP = (double*)calloc(4 * (size_t)niR, sizeof(double));
+  const double rc_sq = e->cutoff * e->cutoff;
+  double bs[3];
+  box_size(box, bs);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int64_t i = 0; i < natoms; ++i) {
+      const double* pi = pos + 3 * i;
+      const int* nb = nl->nbrs + nl->offsets[i];
+      for (int k = 0; k < nl->counts[i]; ++k) {
+        const int j = nb[k];
+        double d[3] = {pi[0] - pos[3 * j], pi[1] - pos[3 * j + 1], pi[2] - pos[3 * j + 2]};
+        orc_minimum_image(d, bs, box->periodic);
+        const double r2 = length_sq(d);
+        if (r2 >= rc_sq) continue;
+        const double r = sqrt(r2);
+        double t;
+        const int q = spline_locate(sR, r, &t);
+        if (pass == 0) {
+          const double w = -2.0 * ((f[3 * i] - f[3 * j]) * d[0] + (f[3 * i + 1] - f[3 * j + 1]) * d[1] +
+                                   (f[3 * i + 2] - f[3 * j + 2]) * d[2]) / r;
+          const double P = fp[i] + fp[j];
+          cP[1 * niR + q] += w;
+          cP[2 * niR + q] += 2.0 * t * w;
+          cP[3 * niR + q] += 3.0 * t * t * w;
+          cR[1 * niR + q] += w * P;
+          cR[2 * niR + q] += 2.0 * t * w * P;
+          cR[3 * niR + q] += 3.0 * t * t * w * P;
+          const double drho = orc_spline_eval_deriv(sR, r);
+          lam[i] += w * drho;
+          lam[j] += w * drho;
+        } else {
+          /* density values: mu_i rho(r_ij) enters for both ends */
+          double ti;
+          const int qi = spline_locate(sF, rho[i], &ti);
+          double tj;
+          const int qj = spline_locate(sF, rho[j], &tj);
+          const double mu = lam[i] * (2.0 * sF->c[qi] + 6.0 * sF->d[qi] * ti) +
+                            lam[j] * (2.0 * sF->c[qj] + 6.0 * sF->d[qj] * tj);
+          cR[0 * niR + q] += mu;
+          cR[1 * niR + q] += mu * t;
+          cR[2 * niR + q] += mu * t * t;
+          cR[3 * niR + q] += mu * t * t * t;
+        }
+      }
+    }
+  }
+  for (int64_t i = 0; i < natoms; ++i) {
+    double t;
+    const int q = spline_locate(sF, rho[i], &t);
+    cF[1 * niF + q] += lam[i];
+    cF[2 * niF + q] += 2.0 * t * lam[i];
+    cF[3 * niF + q] += 3.0 * t * t * lam[i];
+  }
+  orc_spline_coef_grad_to_values(sF->n, sF->dr, cF, cF + niF, cF + 2 * niF, cF + 3 * niF, gF);
+  orc_spline_coef_grad_to_values(sR->n, sR->dr, cR, cR + niR, cR + 2 * niR, cR + 3 * niR, grho);
+  orc_spline_coef_grad_to_values(sP->n, sP->dr, cP, cP + niR, cP + 2 * niR, cP + 3 * niR, gphi);
+  if (gcoef) {
+    memcpy(gcoef, cF, 4 * (size_t)niF * sizeof(double));
+    memcpy(gcoef + 4 * (size_t)niF, cR, 4 * (size_t)niR * sizeof(double));
+    memcpy(gcoef + 4 * (size_t)niF + 4 * (size_t)niR, cP, 4 * (size_t)niR * sizeof(double));
+  }
+  free(f);
+  free(rho);
+  free(fp);
+  free(lam);
+  free(types);
+  free(cF);
+  free(cR);
+  free(cP);
+  return L;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Velocity Verlet: src/integrator/velocity_verlet.cpp:11-48                */
 /* ------------------------------------------------------------------------ */
 

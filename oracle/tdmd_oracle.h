@@ -141,6 +141,38 @@ double orc_eam_forces(const orc_eam* e, const double* pos, const int* types,
                       double* embed_energy_out);
 void orc_eam_free(orc_eam* e);
 
+/* ---- force matching (no reference counterpart: the gradient the fitting
+ * harness needs, stated here in FP64 as the checker of the GPU adjoint).
+ * L = sum_i |F_i - T_i|^2 with F from orc_morse_forces / orc_eam_forces over
+ * the half list `nl`. Pinned by central finite differences of L. ---- */
+
+/* dL/d(D, alpha, r0) -> grad[3]; returns L. forces_out (3N) may be NULL. */
+double orc_fm_morse(const double* pos, int64_t natoms, const orc_box* box,
+                    const orc_nlist* nl, double D, double alpha, double r0,
+                    double rc, const double* target, double grad[3],
+                    double* forces_out);
+
+/* Single-element EAM whose parameters are the spline knot values as EamAlloy
+ * holds them (F[nrho], rho[nr], phi[nr] with phi = (r phi)/r). */
+int orc_eam_from_values(orc_eam* e, int nrho, double drho, int nr, double dr,
+                        double cutoff, const double* F, const double* rho,
+                        const double* phi);
+/* dL/d(F knots, rho knots, phi knots) -> gF[nrho], grho[nr], gphi[nr], through
+ * the FD-Hermite coefficients (eam_alloy.cpp:19-50); returns L.
+ * gcoef (optional, 4*(nrho-1) + 8*(nr-1) doubles) receives the gradient with
+ * respect to the cubic coefficients: F {a,b,c,d}, rho {a,b,c,d},
+ * phi {a,b,c,d}, each block [4][n-1]. */
+double orc_fm_eam(const orc_eam* e, const double* pos, int64_t natoms,
+                  const orc_box* box, const orc_nlist* nl, const double* target,
+                  double* gF, double* grho, double* gphi, double* gcoef,
+                  double* forces_out);
+
+/* Transpose of the FD-Hermite map for one spline: knot-value gradient gv[n]
+ * from coefficient gradients ga, gb, gc, gd [n-1]. */
+void orc_spline_coef_grad_to_values(int n, double dr, const double* ga,
+                                    const double* gb, const double* gc,
+                                    const double* gd, double* gv);
+
 /* ---- velocity Verlet: src/integrator/velocity_verlet.cpp:11-48 ---- */
 void orc_half_kick(double* vel, const double* forces, const int* types,
                    const double* masses, int64_t natoms, double dt);

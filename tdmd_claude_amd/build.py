@@ -29,6 +29,7 @@ SOURCES = [
     "forces_csr.cpp",
     "integrate.cpp",
     "scheduler.cpp",
+    "fm.cpp",
 ]
 
 ARCH = "gfx950"

@@ -165,6 +165,13 @@ _PROTOS = {
     "tdmd_sched_thermo": (C.c_int, [_VP, _PD]),
     "tdmd_sched_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
     "tdmd_sched_time_passes": (C.c_int, [_VP, _I32, C.POINTER(C.c_float)]),
+    "tdmd_fm_create": (C.c_int, [C.POINTER(_VP), _I32, C.POINTER(Box), _I32, _I32, _PD, _D]),
+    "tdmd_fm_destroy": (C.c_int, [_VP]),
+    "tdmd_fm_info": (C.c_int, [_VP, _PI32, C.POINTER(_I64)]),
+    "tdmd_fm_set_targets": (C.c_int, [_VP, _PD]),
+    "tdmd_fm_morse": (C.c_int, [_VP, _PD, _D, _PD, _PD, _PD]),
+    "tdmd_fm_eam": (C.c_int, [_VP, _I32, _D, _I32, _D, _D, _VP, _VP, _VP, _VP, _VP]),
+    "tdmd_fm_eam_coef_grad": (C.c_int, [_VP, _PD]),
 }
 
 _lib = None

@@ -253,3 +253,84 @@ def test_nhc_conserved_energy_single_thermostat():
             assert exchanged > 0.5
             assert dev[1] < 0.02 * exchanged
     assert dev[3] > 5 * dev[1]
+
+
+# ---------------------------------------------------------------------------
+# Force matching: the oracle's analytic gradients against central finite
+# differences of the oracle's own loss (FP64). These pin the checker of the
+# GPU adjoint; a dropped term (e.g. the density chain through F'') or a
+# wrong factor fails them.
+# ---------------------------------------------------------------------------
+
+def _fm_system():
+    s = I.fcc_lattice(3, 3, 3, 3.615)
+    I.displace_gaussian(s, 0.05, 5)
+    return s
+
+
+def test_fm_morse_gradient_finite_difference():
+    s = _fm_system()
+    half = O.HalfList(s.pos, s.lo, s.hi, 6.0, 0.0)
+    _, target = O.morse_forces(half, 0.36, 1.30, 2.90, 6.0)
+    th = np.array([0.3429, 1.3588, 2.866])
+    L, g, f = O.fm_morse(half, *th, 6.0, target)
+    _, f_ref = O.morse_forces(half, *th, 6.0)
+    assert np.array_equal(f, f_ref)
+    assert L == pytest.approx(float(np.sum((f_ref - target) ** 2)), rel=1e-14)
+    for k in range(3):
+        h = 1e-6
+        tp, tm = th.copy(), th.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (O.fm_morse(half, *tp, 6.0, target)[0] - O.fm_morse(half, *tm, 6.0, target)[0]) / (2 * h)
+        assert g[k] == pytest.approx(fd, rel=1e-7)
+
+
+def _eam_values(t):
+    F = np.asarray(t.F).reshape(-1)[: t.nrho]
+    rho = np.asarray(t.rho).reshape(-1)[: t.nr]
+    rphi = np.asarray(t.rphi).reshape(-1)[: t.nr]
+    phi = np.zeros(t.nr)
+    phi[1:] = rphi[1:] / (np.arange(1, t.nr) * t.dr)
+    return F, rho, phi
+
+
+def test_fm_eam_gradient_finite_difference():
+    s = _fm_system()
+    t = I.johnson_setfl(nr=500, nrho=500)
+    F, rho, phi = _eam_values(t)
+    half = O.HalfList(s.pos, s.lo, s.hi, t.cutoff, 0.0)
+    tgt_model = O.EamValues(F * 1.03, rho * 0.98, phi * 1.05, t.drho, t.dr, t.cutoff)
+    target = tgt_model.forces(half, np.ones(len(s.pos), dtype=np.int32))["forces"]
+    model = O.EamValues(F, rho, phi, t.drho, t.dr, t.cutoff)
+    L, G, f = O.fm_eam(model, half, target)
+    assert L == pytest.approx(float(np.sum((f - target) ** 2)), rel=1e-14)
+    rng = np.random.default_rng(3)
+    base = {"F": F, "rho": rho, "phi": phi}
+    for name in ("F", "rho", "phi"):
+        gv = G[name]
+        live = np.nonzero(np.abs(gv) > 1e-3 * np.abs(gv).max())[0]
+        for k in rng.choice(live, 3, replace=False):
+            h = 1e-5 * max(1.0, abs(base[name][k]))
+
+            def loss(delta):
+                v = {kk: vv.copy() for kk, vv in base.items()}
+                v[name][k] += delta
+                m = O.EamValues(v["F"], v["rho"], v["phi"], t.drho, t.dr, t.cutoff)
+                return O.fm_eam(m, half, target)[0]
+
+            fd = (loss(h) - loss(-h)) / (2 * h)
+            # F'' of the FD-Hermite spline jumps at knots, which limits the
+            # finite difference for the density knots
+            assert gv[k] == pytest.approx(fd, rel=2e-5 if name == "rho" else 1e-8), (name, k)
+
+
+def test_spline_value_gradient_transpose():
+    # <H^T gc, dv> == <gc, H dv> for the FD-Hermite map H (values -> coefficients)
+    rng = np.random.default_rng(11)
+    n, dr = 37, 0.1
+    gc = rng.normal(size=(4, n - 1))
+    dv = rng.normal(size=n)
+    gv = O.spline_coef_grad_to_values(n, dr, *gc)
+    coef = np.array(O.spline_coeffs(dv, dr))
+    assert float(gv @ dv) == pytest.approx(float(np.sum(gc * coef)), rel=1e-12)
