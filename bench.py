@@ -84,6 +84,103 @@ def bytes_per_atom(precision: str, mirror_fp64: bool, nbar: float) -> dict:
     return dict(density=density, force=force, step=density + force)
 
 
+def bench_morse_c2(steps: int) -> dict:
+    """32,000-atom Morse Cu (20^3 cells, 300 K, +-0.02 A jitter), NVE, skin/2
+    displacement rebuild: mixed and FP64 throughput and the mixed energy drift."""
+    s = I.fcc_lattice(20, 20, 20, 3.615)
+    I.init_velocities(s, 300.0, 42)
+    I.jitter_uniform(s, 0.02, 4242)
+    morse = (0.3429, 1.3588, 2.866, 6.0)
+    out = {"workload": "Morse Cu 20x20x20 FCC cells (32,000 atoms), NVE, dt 1 fs, skin 1 A, "
+                       "skin/2 displacement rebuild", "steps": steps}
+    for prec in ("mixed", "fp64"):
+        cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=prec,
+                                 rebuild_policy="displacement")
+        sched = FastPipelineScheduler.for_system(s, morse, cfg)
+        sched.run_until(0)
+        th0 = sched.thermo()
+        wall, _ = timed_run(sched, 100, steps)
+        th1 = sched.thermo()
+        e0 = (th0["pe"] + th0["ke"]) / s.natoms
+        e1 = (th1["pe"] + th1["ke"]) / s.natoms
+        out[prec] = {"value": round(s.natoms * steps / wall / 1e6, 2),
+                     "ms_per_step": round(1000 * wall / steps, 4),
+                     "energy_change_ev_per_atom": e1 - e0,
+                     "drift_ev_per_atom_per_ns": (e1 - e0) / ((steps + 100) * 1e-6),
+                     "rebuilds": sched.stats().rebuilds}
+        sched.close()
+    out["mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
+def bench_force_matching(iters: int) -> dict:
+    """Force matching over 256 snapshots of 10^3-cell Cu (4,000 atoms, N(0, 0.05^2)
+    displacements, seeds 5..260): loss+adjoint evaluation time for Morse and for the
+    5000-point EAM knot values (mixed), and an Adam fit of `iters` iterations from
+    the unperturbed tables to targets of a perturbed Johnson parameter set."""
+    import torch
+
+    from tdmd_claude_amd import fitting
+
+    base, snaps = fitting.fc_snapshots(10, 256, 0.05, 5)
+    t0 = time.perf_counter()
+    fm_m = fitting.ForceMatchSet(base.lo, base.hi, snaps, 6.0, "mixed")
+    setup = time.perf_counter() - t0
+    morse_t = (0.36, 1.30, 2.90)
+    fm_m.set_targets(fm_m.morse_forces(morse_t, 6.0))
+    theta = (0.3429, 1.3588, 2.866)
+    fm_m.morse(theta, 6.0)
+    n_eval = 20
+    capi.call("tdmd_device_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(n_eval):
+        fm_m.morse(theta, 6.0)
+    morse_ms = 1000 * (time.perf_counter() - t0) / n_eval
+    x0 = torch.tensor(theta, dtype=torch.float64)
+    _, mlosses = fitting.fit(lambda p: fitting.MorseForceMatch.apply(p, fm_m, 6.0), x0,
+                             iters=iters, lr=5e-3)
+    npairs_m = fm_m.npairs()
+    fm_m.close()
+
+    tab = I.johnson_setfl()
+    values, grid = fitting.eam_values(tab)
+    pt = I.JohnsonParams(beta=4.5397 * 1.02, E0=3.4842 * 0.98, D_p=0.0592 * 1.03)
+    vt, gt = fitting.eam_values(I.johnson_setfl(pt))
+    fm_e = fitting.ForceMatchSet(base.lo, base.hi, snaps, grid.cutoff, "mixed")
+    dev = torch.device("cuda")
+    _, _, ftgt = fm_e.eam(torch.tensor(vt, dtype=torch.float64, device=dev), gt,
+                          want_loss=False, want_grad=False, want_forces=True)
+    fm_e.set_targets(ftgt.cpu().numpy())
+    v0 = torch.tensor(values, dtype=torch.float64, device=dev)
+    fm_e.eam(v0, grid)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_eval):
+        fm_e.eam(v0, grid)
+    torch.cuda.synchronize()
+    eam_ms = 1000 * (time.perf_counter() - t0) / n_eval
+    t0 = time.perf_counter()
+    _, elosses = fitting.fit(lambda v: fitting.EamForceMatch.apply(v, fm_e, grid), v0,
+                             iters=iters, lr=2e-4)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    npairs_e = fm_e.npairs()
+    fm_e.close()
+    natoms = fm_e.ntot
+    return {
+        "workload": "256 snapshots x 4,000-atom FCC Cu (sigma 0.05 A), mixed, forces + "
+                    "adjoint gradient per evaluation",
+        "atoms": natoms, "list_setup_s": round(setup, 3),
+        "morse": {"eval_ms": round(morse_ms, 3), "pairs": npairs_m,
+                  "matom_evals_per_s": round(natoms / morse_ms / 1e3, 1),
+                  "adam_iters": iters, "loss_start": mlosses[0], "loss_end": mlosses[-1]},
+        "eam": {"eval_ms": round(eam_ms, 3), "pairs": npairs_e,
+                "matom_evals_per_s": round(natoms / eam_ms / 1e3, 1),
+                "parameters": grid.nvalues, "adam_iters": iters,
+                "fit_s": round(fit_s, 3), "loss_start": elosses[0], "loss_end": elosses[-1]},
+    }
+
+
 def nve_drift(system: I.System, tables, precision: str, steps: int, every: int) -> dict:
     """Mixed NVE run from the given state: slope of total energy per atom."""
     cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=precision,
@@ -114,6 +211,10 @@ def main() -> int:
     ap.add_argument("--nve-steps", type=int, default=2000)
     ap.add_argument("--mirror-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c2-steps", type=int, default=10000,
+                    help="Morse 32k-atom NVE steps per precision (0: skip)")
+    ap.add_argument("--fm-iters", type=int, default=200,
+                    help="force-matching Adam iterations (0: skip the force-matching leg)")
     ap.add_argument("--profile", action="store_true",
                     help="short mixed run only (for rocprofv3)")
     args = ap.parse_args()
@@ -197,6 +298,19 @@ def main() -> int:
     drift = nve_drift(state, tables, "mixed", args.nve_steps, max(args.nve_steps // 10, 1))
     log(f"NVE drift: {drift['drift_ev_per_atom_per_ns']:.3e} eV/atom/ns")
 
+    # ---- other workloads of the path (not the metric) ----
+    extras = {}
+    if args.c2_steps > 0:
+        extras["morse_32k"] = bench_morse_c2(args.c2_steps)
+        log(f"Morse 32k: mixed {extras['morse_32k']['mixed']['value']} / fp64 "
+            f"{extras['morse_32k']['fp64']['value']} Matom-steps/s")
+    if args.fm_iters > 0:
+        extras["force_matching"] = bench_force_matching(args.fm_iters)
+        fmx = extras["force_matching"]
+        log(f"force matching: Morse eval {fmx['morse']['eval_ms']} ms, EAM eval "
+            f"{fmx['eam']['eval_ms']} ms, EAM fit {fmx['eam']['fit_s']} s "
+            f"(loss {fmx['eam']['loss_start']:.4e} -> {fmx['eam']['loss_end']:.4e})")
+
     # ---- CPU baseline leg: FP64 oracle driver (also the force checker) ----
     cpu = None
     force_err = None
@@ -266,6 +380,7 @@ def main() -> int:
             "density_pass_ms": round(t_dens, 5),
         },
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
     return 0

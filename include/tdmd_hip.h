@@ -292,6 +292,22 @@ int tdmd_sched_create_eam(tdmd_sched** out, const tdmd_box* box,
 int tdmd_sched_create_eam_tables(tdmd_sched** out, const tdmd_box* box,
                                  int32_t natoms, const tdmd_eam_tables* tables,
                                  const tdmd_sched_config* cfg);
+/* A batch of nrep independent systems (replicas) advanced together: box
+ * boxes[r] holds natoms[r] atoms; upload/download take the replicas'
+ * arrays concatenated in replica order. Every step pass is one launch over
+ * all replicas (no reference counterpart: the reference runs one system per
+ * scheduler). NVE only; the rebuild test is shared by all replicas. */
+int tdmd_sched_create_eam_batch(tdmd_sched** out, int32_t nrep,
+                                const tdmd_box* boxes, const int32_t* natoms,
+                                int32_t ntypes, int32_t nrho, double drho,
+                                int32_t nr, double dr, double cutoff,
+                                const double* F, const double* rho,
+                                const double* rphi,
+                                const tdmd_sched_config* cfg);
+int tdmd_sched_create_morse_batch(tdmd_sched** out, int32_t nrep,
+                                  const tdmd_box* boxes, const int32_t* natoms,
+                                  const tdmd_morse_params* params,
+                                  const tdmd_sched_config* cfg);
 int tdmd_sched_destroy(tdmd_sched* s);
 /* upload (fast_pipeline_scheduler.cu:49-73). pos, vel, forces: double[3N];
  * masses: double[n_masses] indexed by 1-based type. forces may be NULL. */

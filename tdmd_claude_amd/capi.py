@@ -155,6 +155,11 @@ _PROTOS = {
                                         _I32, _D, _D, _PD, _PD, _PD, C.POINTER(SchedConfig)]),
     "tdmd_sched_create_eam_tables": (C.c_int, [C.POINTER(_VP), C.POINTER(Box), _I32, _VP,
                                                C.POINTER(SchedConfig)]),
+    "tdmd_sched_create_eam_batch": (C.c_int, [C.POINTER(_VP), _I32, C.POINTER(Box), _PI32,
+                                              _I32, _I32, _D, _I32, _D, _D, _PD, _PD, _PD,
+                                              C.POINTER(SchedConfig)]),
+    "tdmd_sched_create_morse_batch": (C.c_int, [C.POINTER(_VP), _I32, C.POINTER(Box), _PI32,
+                                                C.POINTER(MorseParams), C.POINTER(SchedConfig)]),
     "tdmd_sched_destroy": (C.c_int, [_VP]),
     "tdmd_sched_upload": (C.c_int, [_VP, _PD, _PD, _PD, _PI32, _PI32, _PD, _I32]),
     "tdmd_sched_run_until": (C.c_int, [_VP, _I32]),

@@ -56,6 +56,7 @@ __device__ __forceinline__ double min_image_sel(double d, double L, double hL, i
 
 struct MirrorF64 {
   using T = double4;
+  static constexpr bool kMB = false;
   __device__ __forceinline__ static T encode(const MirrorParams&, double x, double y, double z,
                                              double w) {
     return make_double4(x, y, z, w);
@@ -89,6 +90,7 @@ struct MirrorF64 {
 
 struct MirrorFixed {
   using T = int4;
+  static constexpr bool kMB = false;
   // Coordinate in units, reduced mod 2^32 (a periodic image on a periodic
   // axis; an open axis stays in range, see representable()).
   __device__ __forceinline__ static int enc1(double x, double lo, double q) {
@@ -141,6 +143,23 @@ struct MirrorFixed {
   }
 };
 
+// A batch of independent boxes (replicas) advanced together: the same
+// mirror encoding, with each atom's box looked up from its replica id.
+template <class Base>
+struct ManyBoxes : Base {
+  static constexpr bool kMB = true;
+};
+
+// One box of a batch: its mirror parameters and binning grid; its cells are
+// global cells [cell_off, cell_off + n0 n1 n2).
+struct BoxGeo {
+  MirrorParams mp;
+  int n[3];
+  double inv_cs[3];
+  int cell_off;
+  int pad;
+};
+
 // ELL index of the k-th neighbour of atom i.
 __device__ __forceinline__ size_t ell(int i, int k, int cap) {
   return (static_cast<size_t>(i >> 6) * cap + k) * 64 + (i & 63);
@@ -182,7 +201,20 @@ struct StepArgs {
   int32_t* flags;        // [0] rebuild, [1] overflow max, [2] nonfinite
   double* partials;      // per-block FP64 partial sums
   double* nhc;           // device NHC state block
+  // batches of boxes (ManyBoxes mirrors only)
+  const BoxGeo* boxes;   // [nrep]
+  const int32_t* rep;    // [N] replica of each slot (static: the sort keeps
+                         // every replica in its own slot range)
 };
+
+// Mirror parameters of the box atom i belongs to.
+template <class Mir>
+__device__ __forceinline__ const MirrorParams& mp_of(const StepArgs& a, int i) {
+  if constexpr (Mir::kMB)
+    return a.boxes[a.rep[i]].mp;
+  else
+    return a.mp;
+}
 
 }  // namespace sched
 }  // namespace tdmd_hip

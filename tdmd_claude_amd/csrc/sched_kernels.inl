@@ -65,21 +65,22 @@ __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vsca
   const double vx = vscale * a.vx[i] + factor * f0;
   const double vy = vscale * a.vy[i] + factor * f1;
   const double vz = vscale * a.vz[i] + factor * f2;
-  const double x = wrap1(a.px[i] + a.dt * vx, a.mp.lo[0], a.mp.L[0], a.mp.periodic[0]);
-  const double y = wrap1(a.py[i] + a.dt * vy, a.mp.lo[1], a.mp.L[1], a.mp.periodic[1]);
-  const double z = wrap1(a.pz[i] + a.dt * vz, a.mp.lo[2], a.mp.L[2], a.mp.periodic[2]);
+  const MirrorParams& mp = mp_of<Mir>(a, i);
+  const double x = wrap1(a.px[i] + a.dt * vx, mp.lo[0], mp.L[0], mp.periodic[0]);
+  const double y = wrap1(a.py[i] + a.dt * vy, mp.lo[1], mp.L[1], mp.periodic[1]);
+  const double z = wrap1(a.pz[i] + a.dt * vz, mp.lo[2], mp.L[2], mp.periodic[2]);
   a.vx[i] = vx;
   a.vy[i] = vy;
   a.vz[i] = vz;
   a.px[i] = x;
   a.py[i] = y;
   a.pz[i] = z;
-  static_cast<typename Mir::T*>(a.M)[i] = encode_type<Mir>(a.mp, x, y, z, ti);
-  const double dx = min_image(x - a.bx[i], a.mp.L[0], a.mp.hL[0], a.mp.periodic[0]);
-  const double dy = min_image(y - a.by[i], a.mp.L[1], a.mp.hL[1], a.mp.periodic[1]);
-  const double dz = min_image(z - a.bz[i], a.mp.L[2], a.mp.hL[2], a.mp.periodic[2]);
+  static_cast<typename Mir::T*>(a.M)[i] = encode_type<Mir>(mp, x, y, z, ti);
+  const double dx = min_image(x - a.bx[i], mp.L[0], mp.hL[0], mp.periodic[0]);
+  const double dy = min_image(y - a.by[i], mp.L[1], mp.hL[1], mp.periodic[1]);
+  const double dz = min_image(z - a.bz[i], mp.L[2], mp.hL[2], mp.periodic[2]);
   if (dx * dx + dy * dy + dz * dz > a.skin_half_sq) a.flags[0] = 1;
-  if (!(isfinite(x) && isfinite(y) && isfinite(z) && Mir::representable(a.mp, x, y, z)))
+  if (!(isfinite(x) && isfinite(y) && isfinite(z) && Mir::representable(mp, x, y, z)))
     a.flags[2] = 1;
 }
 
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(kBlock) k_write_mirror(StepArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.natoms) return;
   static_cast<typename Mir::T*>(a.M)[i] =
-      encode_type<Mir>(a.mp, a.px[i], a.py[i], a.pz[i], a.types[i] - 1);
+      encode_type<Mir>(mp_of<Mir>(a, i), a.px[i], a.py[i], a.pz[i], a.types[i] - 1);
 }
 
 template <class R>
@@ -295,12 +296,13 @@ k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
   double e = 0.0;
   if (i < a.natoms) {
     const T mi = M[i];
+    const MirrorParams& mpi = mp_of<Mir>(a, i);
     const int ti = Mir::type_of(mi);
     const int n = a.cnt[i];
     double rho = 0.0;
     for_neighbours<T, kPairUnroll>(a, i, n, M, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         rho += static_cast<double>(spline_value<R>(sm[a.ntypes + Mir::type_of(mj)], coeff, pr.r));
@@ -327,13 +329,14 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
   double red = 0.0;
   if (i < a.natoms) {
     const T mi = M2[i];
+    const MirrorParams& mpi = mp_of<Mir>(a, i);
     const R fpi = Mir::template w_real<R>(mi);
     const int ti = kMulti ? a.types[i] - 1 : 0;
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
     for_neighbours<T, kPairUnroll>(a, i, n, M2, [&](int j, const T& mj, bool in_row) {
       double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         const int tj = kMulti ? a.types[j] - 1 : 0;
@@ -376,11 +379,12 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
   double red = 0.0;
   if (i < a.natoms) {
     const T mi = M[i];
+    const MirrorParams& mpi = mp_of<Mir>(a, i);
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
     for_neighbours<T, kPairUnroll>(a, i, n, M, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < p.rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         R e;
@@ -438,23 +442,55 @@ __global__ void __launch_bounds__(kBlock) k_zero_cells(const int32_t* flags, Gri
   g.cursor[r] = 0;
 }
 
-__device__ __forceinline__ void cell_coords(const StepArgs& a, const GridArgs& g, double x,
-                                            double y, double z, int c[3]) {
+// Binning grid of atom i's box; global cell id = off + (c2 n1 + c1) n0 + c0.
+struct CellGrid {
+  int n[3];
+  double inv_cs[3];
+  double lo[3];
+  int off;
+};
+
+template <class Mir>
+__device__ __forceinline__ CellGrid grid_of(const StepArgs& a, const GridArgs& g, int i) {
+  CellGrid cg;
+  if constexpr (Mir::kMB) {
+    const BoxGeo& b = a.boxes[a.rep[i]];
+    for (int d = 0; d < 3; ++d) {
+      cg.n[d] = b.n[d];
+      cg.inv_cs[d] = b.inv_cs[d];
+      cg.lo[d] = b.mp.lo[d];
+    }
+    cg.off = b.cell_off;
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      cg.n[d] = g.n[d];
+      cg.inv_cs[d] = g.inv_cs[d];
+      cg.lo[d] = a.mp.lo[d];
+    }
+    cg.off = 0;
+  }
+  return cg;
+}
+
+__device__ __forceinline__ void cell_coords(const CellGrid& cg, double x, double y, double z,
+                                            int c[3]) {
   const double p[3] = {x, y, z};
   for (int d = 0; d < 3; ++d) {
-    int v = static_cast<int>((p[d] - a.mp.lo[d]) * g.inv_cs[d]);
-    v = v >= g.n[d] ? g.n[d] - 1 : v;
+    int v = static_cast<int>((p[d] - cg.lo[d]) * cg.inv_cs[d]);
+    v = v >= cg.n[d] ? cg.n[d] - 1 : v;
     c[d] = v < 0 ? 0 : v;
   }
 }
 
+template <class Mir>
 __global__ void __launch_bounds__(kBlock) k_bin(StepArgs a, GridArgs g) {
   if (!a.flags[0]) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.natoms) return;
   int c[3];
-  cell_coords(a, g, a.px[i], a.py[i], a.pz[i], c);
-  const int rank = g.rank_of[(c[2] * g.n[1] + c[1]) * g.n[0] + c[0]];
+  const CellGrid cg = grid_of<Mir>(a, g, i);
+  cell_coords(cg, a.px[i], a.py[i], a.pz[i], c);
+  const int rank = g.rank_of[cg.off + (c[2] * cg.n[1] + c[1]) * cg.n[0] + c[0]];
   g.key[i] = rank;
   atomicAdd(&g.ccount[rank], 1);
 }
@@ -558,7 +594,7 @@ __global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, Per
   const int ty = t.types[s];
   const_cast<int32_t*>(a.types)[s] = ty;
   g.orig[s] = t.orig[s];
-  static_cast<typename Mir::T*>(a.M)[s] = Mir::encode(a.mp, x, y, z, ty - 1);
+  static_cast<typename Mir::T*>(a.M)[s] = Mir::encode(mp_of<Mir>(a, s), x, y, z, ty - 1);
 }
 
 // ELL list over the sorted atoms. Inclusion exactly as the CPU list
@@ -572,7 +608,7 @@ __global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, Per
 // nearly every lane of a wave, and whole waves skip the pair work of the
 // shell. Near entries are written from the front and shell entries from the
 // back of the row; the shell block is then moved down behind the near block.
-template <bool kCount>
+template <bool kCount, class Mir>
 __global__ void __launch_bounds__(kBlock)
 k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, double near_sq, int32_t* nbr,
             int32_t* cnt, int32_t* maxcnt) {
@@ -582,24 +618,26 @@ k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, double near_sq, int32_t* n
   const int cap = a.cap;
   if (i < a.natoms) {
     const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
+    const MirrorParams& mp = mp_of<Mir>(a, i);
+    const CellGrid cg = grid_of<Mir>(a, g, i);
     int c[3];
-    cell_coords(a, g, xi, yi, zi, c);
+    cell_coords(cg, xi, yi, zi, c);
     for (int dz = -1; dz <= 1; ++dz) {
       int nz = c[2] + dz;
-      nz = nz < 0 ? nz + g.n[2] : (nz >= g.n[2] ? nz - g.n[2] : nz);
+      nz = nz < 0 ? nz + cg.n[2] : (nz >= cg.n[2] ? nz - cg.n[2] : nz);
       for (int dy = -1; dy <= 1; ++dy) {
         int ny = c[1] + dy;
-        ny = ny < 0 ? ny + g.n[1] : (ny >= g.n[1] ? ny - g.n[1] : ny);
+        ny = ny < 0 ? ny + cg.n[1] : (ny >= cg.n[1] ? ny - cg.n[1] : ny);
         for (int dx = -1; dx <= 1; ++dx) {
           int nx = c[0] + dx;
-          nx = nx < 0 ? nx + g.n[0] : (nx >= g.n[0] ? nx - g.n[0] : nx);
-          const int r = g.rank_of[(nz * g.n[1] + ny) * g.n[0] + nx];
+          nx = nx < 0 ? nx + cg.n[0] : (nx >= cg.n[0] ? nx - cg.n[0] : nx);
+          const int r = g.rank_of[cg.off + (nz * cg.n[1] + ny) * cg.n[0] + nx];
           const int start = g.cstart[r], end = start + g.ccount[r];
           for (int j = start; j < end; ++j) {
             if (j == i) continue;
-            const double ddx = min_image(a.px[j] - xi, a.mp.L[0], a.mp.hL[0], a.mp.periodic[0]);
-            const double ddy = min_image(a.py[j] - yi, a.mp.L[1], a.mp.hL[1], a.mp.periodic[1]);
-            const double ddz = min_image(a.pz[j] - zi, a.mp.L[2], a.mp.hL[2], a.mp.periodic[2]);
+            const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
+            const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
+            const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
             const double r2 = r2_unfused(ddx, ddy, ddz);
             if (r2 < r_list_sq) {
               if (!kCount && nn + nf < cap) {

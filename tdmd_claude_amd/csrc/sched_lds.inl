@@ -74,11 +74,12 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   double e = 0.0;
   if (i < a.natoms) {
     const T mi = M[i];
+    const MirrorParams& mpi = mp_of<Mir>(a, i);
     const int n = a.cnt[i];
     double rho = 0.0;
     for_neighbours<T, kU>(a, i, n, M, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         R t;
@@ -130,12 +131,13 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   double red = 0.0;
   if (i < a.natoms) {
     const T mi = M2[i];
+    const MirrorParams& mpi = mp_of<Mir>(a, i);
     const R fpi = Mir::template w_real<R>(mi);
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
     for_neighbours<T, kU>(a, i, n, M2, [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, a.mp, &dx, &dy, &dz);
+      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
         const PairR<R> pr = pair_r<R>(r2);
         R t;

@@ -109,6 +109,11 @@ struct tdmd_sched {
   std::map<std::string, hipGraphExec_t> graphs;
   std::map<std::string, hipGraph_t> graph_defs;
   std::map<std::string, int64_t> graph_kernels;
+  // batches of independent boxes (replicas), atoms of replica r in slots
+  // [rep_off[r], rep_off[r+1])
+  bool multibox = false;
+  std::vector<tdmd_hip::sched::BoxGeo> h_boxes;
+  std::vector<int> rep_off;
 };
 
 namespace tdmd_hip {
@@ -246,7 +251,7 @@ struct Engine final : EngineBase {
   int sort_pass(hipStream_t s) override {
     const int gb = grid_for(S->natoms), gc = grid_for(S->g.ncells);
     k_zero_cells<<<gc, kBlock, 0, s>>>(S->a.flags, S->g);
-    k_bin<<<gb, kBlock, 0, s>>>(S->a, S->g);
+    k_bin<Mir><<<gb, kBlock, 0, s>>>(S->a, S->g);
     k_cell_scan<<<1, 1024, 0, s>>>(S->a.flags, S->g);
     k_scatter<<<gb, kBlock, 0, s>>>(S->a, S->g);
     k_order<<<gc, kBlock, 0, s>>>(S->a, S->g);
@@ -258,7 +263,7 @@ struct Engine final : EngineBase {
   }
   int fill_pass(hipStream_t s) override {
     const int gb = grid_for(S->natoms);
-    k_nlist_ell<false><<<gb, kBlock, 0, s>>>(S->a, S->g, S->r_list * S->r_list,
+    k_nlist_ell<false, Mir><<<gb, kBlock, 0, s>>>(S->a, S->g, S->r_list * S->r_list,
                                              S->cutoff * S->cutoff,
                                              const_cast<int32_t*>(S->a.nbr),
                                              const_cast<int32_t*>(S->a.cnt), nullptr);
@@ -268,7 +273,8 @@ struct Engine final : EngineBase {
     return TDMD_OK;
   }
   int count_pass(hipStream_t s, int32_t* d_max) override {
-    k_nlist_ell<true><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a, S->g, S->r_list * S->r_list,
+    k_nlist_ell<true, Mir><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a, S->g,
+                                                                  S->r_list * S->r_list,
                                                              S->cutoff * S->cutoff, nullptr,
                                                              nullptr, d_max);
     TDMD_LAUNCH_CHECK();
@@ -427,16 +433,39 @@ int sched_alloc(tdmd_sched* S) {
   if ((rc = dalloc(S, &S->d_maxcnt, 4))) return rc;
   if ((rc = dalloc(S, &S->d_rebuilds, 1))) return rc;
   if ((rc = dalloc(S, &S->d_scalars, 16))) return rc;
-  // Morton rank of every cell of the binning grid.
-  std::vector<std::pair<uint32_t, int>> order(static_cast<size_t>(g.ncells));
-  for (int z = 0, c = 0; z < g.n[2]; ++z)
-    for (int y = 0; y < g.n[1]; ++y)
-      for (int x = 0; x < g.n[0]; ++x, ++c)
-        order[static_cast<size_t>(c)] = {spread3(x) | (spread3(y) << 1) | (spread3(z) << 2), c};
-  std::sort(order.begin(), order.end());
+  // Morton rank of every cell of each box's grid; a box's cells take the
+  // global ranks [cell_off, cell_off + ncells_box), so the sort keeps every
+  // box in its own slot range.
   std::vector<int32_t> h_rank(static_cast<size_t>(g.ncells));
-  for (int r = 0; r < g.ncells; ++r) h_rank[static_cast<size_t>(order[static_cast<size_t>(r)].second)] = r;
+  const int nbox = S->multibox ? static_cast<int>(S->h_boxes.size()) : 1;
+  for (int b = 0; b < nbox; ++b) {
+    const int* n = S->multibox ? S->h_boxes[static_cast<size_t>(b)].n : g.n;
+    const int off = S->multibox ? S->h_boxes[static_cast<size_t>(b)].cell_off : 0;
+    const int nc = n[0] * n[1] * n[2];
+    std::vector<std::pair<uint32_t, int>> order(static_cast<size_t>(nc));
+    for (int z = 0, c = 0; z < n[2]; ++z)
+      for (int y = 0; y < n[1]; ++y)
+        for (int x = 0; x < n[0]; ++x, ++c)
+          order[static_cast<size_t>(c)] = {spread3(x) | (spread3(y) << 1) | (spread3(z) << 2), c};
+    std::sort(order.begin(), order.end());
+    for (int r = 0; r < nc; ++r)
+      h_rank[static_cast<size_t>(off + order[static_cast<size_t>(r)].second)] = off + r;
+  }
   TDMD_HIP_TRY(hipMemcpy(rank_of, h_rank.data(), h_rank.size() * 4, hipMemcpyHostToDevice));
+  if (S->multibox) {
+    BoxGeo* boxes;
+    int32_t* rep;
+    if ((rc = dalloc(S, &boxes, S->h_boxes.size()))) return rc;
+    if ((rc = dalloc(S, &rep, N))) return rc;
+    std::vector<int32_t> h_rep(static_cast<size_t>(N));
+    for (size_t b = 0; b < S->h_boxes.size(); ++b)
+      for (int i = S->rep_off[b]; i < S->rep_off[b + 1]; ++i) h_rep[static_cast<size_t>(i)] = static_cast<int32_t>(b);
+    TDMD_HIP_TRY(hipMemcpy(boxes, S->h_boxes.data(), S->h_boxes.size() * sizeof(BoxGeo),
+                           hipMemcpyHostToDevice));
+    TDMD_HIP_TRY(hipMemcpy(rep, h_rep.data(), h_rep.size() * 4, hipMemcpyHostToDevice));
+    a.boxes = boxes;
+    a.rep = rep;
+  }
   TDMD_HIP_TRY(hipMemset(g.ccount, 0, static_cast<size_t>(g.ncells) * 4));
   TDMD_HIP_TRY(hipMemset(g.cursor, 0, static_cast<size_t>(g.ncells) * 4));
   TDMD_HIP_TRY(hipMemset(g.done, 0, 16));
@@ -449,49 +478,83 @@ int sched_alloc(tdmd_sched* S) {
   return TDMD_OK;
 }
 
-int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched_config& cfg) {
-  if (natoms <= 0) return fail(TDMD_E_INVALID, "scheduler: natoms must be positive");
+// Mirror parameters and binning grid of one box.
+int setup_box(const tdmd_box& box, double r_list, MirrorParams* mp, int n[3], double inv_cs[3]) {
+  for (int d = 0; d < 3; ++d) {
+    mp->lo[d] = box.lo[d];
+    mp->L[d] = box.hi[d] - box.lo[d];
+    mp->hL[d] = 0.5 * mp->L[d];
+    mp->periodic[d] = box.periodic[d] ? 1 : 0;
+    if (!(mp->L[d] > 0)) return fail(TDMD_E_INVALID, "scheduler: box must have positive size");
+  }
+  // 32-bit fixed point (MirrorFixed): a periodic axis spans exactly 2^32
+  // units; an open axis gets unit W 2^-30, W = max(L, r_list), so any listed
+  // pair difference stays far below 2^31 units.
+  for (int d = 0; d < 3; ++d) {
+    const double W = mp->periodic[d] ? mp->L[d] : std::max(mp->L[d], r_list);
+    const int e = mp->periodic[d] ? 32 : 30;
+    mp->q[d] = std::ldexp(1.0, e) / W;
+    mp->unit[d] = std::ldexp(W, -e);
+  }
+  // Binning grid: cells no smaller than r_list (with a relative margin so
+  // rounding in the bin computation can never push a neighbour out of the
+  // 27-cell stencil), at least 3 per axis (cell_list.cpp:22-27).
+  for (int d = 0; d < 3; ++d) {
+    n[d] = std::max(3, static_cast<int>(std::floor(mp->L[d] / (r_list * (1.0 + 1e-9)))));
+    inv_cs[d] = static_cast<double>(n[d]) / mp->L[d];
+    if (n[d] > 1024) return fail(TDMD_E_INVALID, "scheduler: more than 1024 cells per axis");
+  }
+  return TDMD_OK;
+}
+
+// boxes.size() == 1 and !multibox: one system. multibox: a batch of
+// independent boxes with counts[r] atoms each, concatenated in box order.
+int sched_setup(tdmd_sched* S, const std::vector<tdmd_box>& boxes, const std::vector<int>& counts,
+                const tdmd_sched_config& cfg, bool multibox) {
+  int64_t total = 0;
+  for (int c : counts) {
+    if (c <= 0) return fail(TDMD_E_INVALID, "scheduler: natoms must be positive");
+    total += c;
+  }
+  if (boxes.empty() || boxes.size() != counts.size() || total > (1LL << 30))
+    return fail(TDMD_E_INVALID, "scheduler: bad box / atom counts");
+  const int natoms = static_cast<int>(total);
   if (!(cfg.dt > 0) || !(cfg.r_skin >= 0) || cfg.rebuild_every < 1)
     return fail(TDMD_E_INVALID, "scheduler: bad dt / r_skin / rebuild_every");
   if (cfg.precision != TDMD_FP64 && cfg.precision != TDMD_MIXED)
     return fail(TDMD_E_INVALID, "scheduler: bad precision");
   if (cfg.t_target > 0 && (cfg.nhc_length < 1 || cfg.nhc_length > 8 || !(cfg.t_period > 0)))
     return fail(TDMD_E_INVALID, "scheduler: NHC chain length must be 1..8, t_period > 0");
+  if (multibox && cfg.t_target > 0)
+    return fail(TDMD_E_INVALID, "scheduler: a batch of boxes runs NVE (one thermostat per box "
+                                "is not provided)");
   if (cfg.pair_unroll != 0 && cfg.pair_unroll != 4 && cfg.pair_unroll != 8)
     return fail(TDMD_E_INVALID, "scheduler: pair_unroll must be 0 (default), 4 or 8");
   if (cfg.lds_budget < 0) return fail(TDMD_E_INVALID, "scheduler: lds_budget must be >= 0");
-  S->box = box;
+  S->box = boxes[0];
   S->natoms = natoms;
   S->cfg = cfg;
+  S->multibox = multibox;
   S->nblocks = round_up(grid_for(natoms), 8);
   S->r_list = S->cutoff + cfg.r_skin;
-  MirrorParams& mp = S->a.mp;
-  for (int d = 0; d < 3; ++d) {
-    mp.lo[d] = box.lo[d];
-    mp.L[d] = box.hi[d] - box.lo[d];
-    mp.hL[d] = 0.5 * mp.L[d];
-    mp.periodic[d] = box.periodic[d] ? 1 : 0;
-    if (!(mp.L[d] > 0)) return fail(TDMD_E_INVALID, "scheduler: box must have positive size");
-  }
-  // 32-bit fixed point (MirrorFixed): a periodic axis spans exactly 2^32
-  // units; an open axis gets unit W 2^-30, W = max(L, r_list), so any listed
-  // pair difference stays far below 2^31 units.
-  for (int d = 0; d < 3; ++d) {
-    const double W = mp.periodic[d] ? mp.L[d] : std::max(mp.L[d], S->r_list);
-    const int e = mp.periodic[d] ? 32 : 30;
-    mp.q[d] = std::ldexp(1.0, e) / W;
-    mp.unit[d] = std::ldexp(W, -e);
-  }
-  // Binning grid: cells no smaller than r_list (with a relative margin so
-  // rounding in the bin computation can never push a neighbour out of the
-  // 27-cell stencil), at least 3 per axis (cell_list.cpp:22-27).
   GridArgs& g = S->g;
-  for (int d = 0; d < 3; ++d) {
-    g.n[d] = std::max(3, static_cast<int>(std::floor(mp.L[d] / (S->r_list * (1.0 + 1e-9)))));
-    g.inv_cs[d] = static_cast<double>(g.n[d]) / mp.L[d];
-    if (g.n[d] > 1024) return fail(TDMD_E_INVALID, "scheduler: more than 1024 cells per axis");
-  }
+  int rc = setup_box(boxes[0], S->r_list, &S->a.mp, g.n, g.inv_cs);
+  if (rc) return rc;
   g.ncells = g.n[0] * g.n[1] * g.n[2];
+  if (multibox) {
+    S->h_boxes.resize(boxes.size());
+    S->rep_off.assign(1, 0);
+    int cells = 0;
+    for (size_t b = 0; b < boxes.size(); ++b) {
+      BoxGeo& bg = S->h_boxes[b];
+      std::memset(&bg, 0, sizeof(bg));
+      if ((rc = setup_box(boxes[b], S->r_list, &bg.mp, bg.n, bg.inv_cs))) return rc;
+      bg.cell_off = cells;
+      cells += bg.n[0] * bg.n[1] * bg.n[2];
+      S->rep_off.push_back(S->rep_off.back() + counts[b]);
+    }
+    g.ncells = cells;
+  }
   S->a.natoms = natoms;
   S->a.ntypes = S->tables ? S->tables->ntypes : 1;
   if (S->a.ntypes < 1 || S->a.ntypes > kMaxTypes ||
@@ -506,16 +569,28 @@ int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched
   S->nhcp.t_period = cfg.t_period;
   S->nhcp.dt = cfg.dt;
   TDMD_HIP_TRY(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
-  int rc = sched_alloc(S);
+  rc = sched_alloc(S);
   if (rc) return rc;
   const bool fixed = cfg.precision == TDMD_MIXED && !cfg.mirror_fp64;
-  if (cfg.precision == TDMD_FP64)
+  if (multibox) {
+    if (cfg.precision == TDMD_FP64)
+      S->eng.reset(new Engine<double, ManyBoxes<MirrorF64>>(S));
+    else if (fixed)
+      S->eng.reset(new Engine<float, ManyBoxes<MirrorFixed>>(S));
+    else
+      S->eng.reset(new Engine<float, ManyBoxes<MirrorF64>>(S));
+  } else if (cfg.precision == TDMD_FP64) {
     S->eng.reset(new Engine<double, MirrorF64>(S));
-  else if (fixed)
+  } else if (fixed) {
     S->eng.reset(new Engine<float, MirrorFixed>(S));
-  else
+  } else {
     S->eng.reset(new Engine<float, MirrorF64>(S));
+  }
   return S->eng->init();
+}
+
+int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched_config& cfg) {
+  return sched_setup(S, std::vector<tdmd_box>{box}, std::vector<int>{natoms}, cfg, false);
 }
 
 void sched_free(tdmd_sched* S) {
@@ -766,6 +841,56 @@ int tdmd_sched_create_eam_tables(tdmd_sched** out, const tdmd_box* box, int32_t 
   }
   S->cutoff = tables->cutoff;
   rc = sched_setup(S, *box, natoms, *cfg);
+  if (rc) {
+    sched_free(S);
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return TDMD_OK;
+}
+
+int tdmd_sched_create_eam_batch(tdmd_sched** out, int32_t nrep, const tdmd_box* boxes,
+                                const int32_t* natoms, int32_t ntypes, int32_t nrho, double drho,
+                                int32_t nr, double dr, double cutoff, const double* F,
+                                const double* rho, const double* rphi,
+                                const tdmd_sched_config* cfg) {
+  if (!out || !boxes || !natoms || !cfg || !F || !rho || !rphi || nrep < 1)
+    return fail(TDMD_E_INVALID, "sched_create_eam_batch: bad arguments");
+  if (ntypes < 1 || ntypes > sched::kMaxTypes)
+    return fail(TDMD_E_INVALID, "sched_create_eam_batch: 1..8 element types");
+  auto* S = new tdmd_sched();
+  S->kind = 1;
+  int rc = tdmd_eam_tables_create(&S->tables, cfg->precision, ntypes, nrho, drho, nr, dr, cutoff,
+                                  F, rho, rphi);
+  if (rc) {
+    delete S;
+    return rc;
+  }
+  S->cutoff = cutoff;
+  rc = sched_setup(S, std::vector<tdmd_box>(boxes, boxes + nrep),
+                   std::vector<int>(natoms, natoms + nrep), *cfg, true);
+  if (rc) {
+    sched_free(S);
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return TDMD_OK;
+}
+
+int tdmd_sched_create_morse_batch(tdmd_sched** out, int32_t nrep, const tdmd_box* boxes,
+                                  const int32_t* natoms, const tdmd_morse_params* params,
+                                  const tdmd_sched_config* cfg) {
+  if (!out || !boxes || !natoms || !params || !cfg || nrep < 1)
+    return fail(TDMD_E_INVALID, "sched_create_morse_batch: bad arguments");
+  if (!(params->rc > 0)) return fail(TDMD_E_INVALID, "sched_create_morse_batch: rc must be > 0");
+  auto* S = new tdmd_sched();
+  S->kind = 0;
+  S->morse = *params;
+  S->cutoff = params->rc;
+  const int rc = sched_setup(S, std::vector<tdmd_box>(boxes, boxes + nrep),
+                             std::vector<int>(natoms, natoms + nrep), *cfg, true);
   if (rc) {
     sched_free(S);
     delete S;

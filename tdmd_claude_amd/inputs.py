@@ -294,6 +294,71 @@ def johnson_setfl(params: JohnsonParams | None = None, nr: int = 5000, nrho: int
         rphi=(r * p.phi(r))[None, :])
 
 
+# Ni/Co/Cr parameters of the three-element Johnson-form alloy: the Cu fit
+# rescaled to a = 3.57 A with element-specific density, embedding and pair
+# strengths (synthetic, for the batched-replica workload; not a fitted alloy).
+ALLOY_NICOCR = {
+    "Ni": dict(f_e=1.00, E0=4.10, D_p=0.0640, beta=4.60, alpha_p=1.52, F2=0.26),
+    "Co": dict(f_e=1.05, E0=4.20, D_p=0.0660, beta=4.50, alpha_p=1.48, F2=0.25),
+    "Cr": dict(f_e=0.92, E0=3.90, D_p=0.0700, beta=4.70, alpha_p=1.45, F2=0.24),
+}
+
+
+def johnson_alloy_setfl(elements=("Ni", "Co", "Cr"), a: float = 3.57, r_c: float = 5.0,
+                        nr: int = 5000, nrho: int = 5000,
+                        rho_max_factor: float = 3.0) -> SetflTables:
+    """Multi-element Johnson-form EAM/alloy tables: per-element F and rho, and
+    per-pair phi with Johnson's mixing rule
+    phi_ab = (f_b / f_a phi_aa + f_a / f_b phi_bb) / 2 (equilibrium densities
+    f at the nearest-neighbour distance), pair blocks in i <= j row order."""
+    scale = a / 3.615
+    ps = []
+    for el in elements:
+        kw = dict(ALLOY_NICOCR[el])
+        ps.append(JohnsonParams(element=el, mass=MASSES[el], a=a, r_p=3.2012 * scale,
+                                r_s=r_c - 0.6, r_c=r_c, **kw))
+    dr = r_c / (nr - 1)
+    drho = rho_max_factor * max(p.rho_e for p in ps) / (nrho - 1)
+    r = np.arange(nr) * dr
+    rho_grid = np.arange(nrho) * drho
+    F = np.stack([p.F(rho_grid) for p in ps])
+    rho = np.stack([p.rho(r) for p in ps])
+    fe = [p.rho(np.array([p.r_e]))[0] for p in ps]
+    rphi = []
+    for i in range(len(ps)):
+        for j in range(i, len(ps)):
+            if i == j:
+                phi = ps[i].phi(r)
+            else:
+                phi = 0.5 * (fe[j] / fe[i] * ps[i].phi(r) + fe[i] / fe[j] * ps[j].phi(r))
+            rphi.append(r * phi)
+    return SetflTables(elements=list(elements), masses=[MASSES[e] for e in elements], nrho=nrho,
+                       drho=drho, nr=nr, dr=dr, cutoff=r_c, F=F, rho=rho, rphi=np.stack(rphi))
+
+
+def alloy_replicas(nrep: int, seed: int, n_min: int = 500, n_max: int = 32000,
+                   a: float = 3.57, t_range=(300.0, 1200.0), elements=("Ni", "Co", "Cr")):
+    """Replicas of random equiatomic FCC alloys: for each, a size drawn
+    log-uniform in [n_min, n_max] and rounded to the nearest 4 n^3 (n >= 3
+    cells, so the 27-cell stencil and the minimum image hold), species
+    uniform at random, Maxwell-Boltzmann velocities at T ~ U(t_range).
+    Returns (list of System, list of temperatures)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    masses = np.array([1.0] + [MASSES[e] for e in elements])
+    systems, temps = [], []
+    for r in range(nrep):
+        target = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        n = max(3, int(round((target / 4.0) ** (1.0 / 3.0))))
+        s = fcc_lattice(n, n, n, a)
+        s.types = rng.integers(1, len(elements) + 1, size=s.natoms).astype(np.int32)
+        s.masses = masses.copy()
+        t = float(rng.uniform(*t_range))
+        init_velocities(s, t, int(rng.integers(0, 2**31 - 1)))
+        systems.append(s)
+        temps.append(t)
+    return systems, temps
+
+
 def write_setfl(path, t: SetflTables, comment: str = "tdmd synthetic EAM/alloy") -> None:
     """setfl text in the layout EamAlloy::read_setfl parses (eam_alloy.cpp:90-192)."""
     def block(vals):

@@ -185,3 +185,62 @@ class FastPipelineScheduler:
             self.close()
         except Exception:
             pass
+
+
+class BatchScheduler(FastPipelineScheduler):
+    """A batch of independent systems (replicas, each in its own box) advanced
+    together: every pass of a step is one launch over all replicas
+    (tdmd_sched_create_{eam,morse}_batch). NVE. Arrays are uploaded and
+    downloaded per replica."""
+
+    def __init__(self, systems: list, potential, cfg: FastPipelineConfig | None = None):
+        self.cfg = cfg or FastPipelineConfig()
+        self.counts = np.array([s.natoms for s in systems], dtype=np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        self.natoms = int(self.offsets[-1])
+        n = len(systems)
+        self._boxes = (capi.Box * n)(*[capi.Box.make(s.lo, s.hi, s.periodic) for s in systems])
+        h = C.c_void_p()
+        c = self.cfg.to_c()
+        if isinstance(potential, SetflTables):
+            t = potential
+            self._F = np.ascontiguousarray(t.F, dtype=np.float64)
+            self._rho = np.ascontiguousarray(t.rho, dtype=np.float64)
+            self._rphi = np.ascontiguousarray(t.rphi, dtype=np.float64)
+            capi.call("tdmd_sched_create_eam_batch", C.byref(h), n, self._boxes, _pi(self.counts),
+                      t.ntypes, t.nrho, t.drho, t.nr, t.dr, t.cutoff, _pd(self._F),
+                      _pd(self._rho), _pd(self._rphi), C.byref(c))
+            self.kind = PotentialKind.Eam
+        else:
+            d, alpha, r0, rc = potential
+            p = capi.MorseParams.make(d, alpha, r0, rc)
+            capi.call("tdmd_sched_create_morse_batch", C.byref(h), n, self._boxes,
+                      _pi(self.counts), C.byref(p), C.byref(c))
+            self.kind = PotentialKind.Morse
+        self._h = h
+
+    @classmethod
+    def for_systems(cls, systems: list, potential, cfg: FastPipelineConfig | None = None):
+        b = cls(systems, potential, cfg)
+        b.upload_systems(systems)
+        return b
+
+    def upload_systems(self, systems: list) -> None:
+        masses = systems[0].masses
+        for s in systems[1:]:
+            if len(s.masses) != len(masses) or not np.array_equal(s.masses, masses):
+                raise ValueError("all replicas must share the per-type masses")
+        pos = np.concatenate([s.pos for s in systems])
+        vel = np.concatenate([s.vel for s in systems])
+        types = np.concatenate([s.types for s in systems])
+        ids = np.concatenate([s.ids for s in systems])
+        self.upload(pos, vel, None, types, ids, masses)
+
+    def download_replicas(self):
+        """[(pos, vel, forces, types, ids) of each replica]."""
+        pos, vel, f, types, ids = self.download()
+        out = []
+        for r in range(len(self.counts)):
+            a, b = self.offsets[r], self.offsets[r + 1]
+            out.append((pos[a:b], vel[a:b], f[a:b], types[a:b], ids[a:b]))
+        return out
