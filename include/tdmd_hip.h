@@ -336,6 +336,14 @@ int tdmd_sched_thermo(tdmd_sched* s, double out[6]);
  * density pass (Morse: kick-drift), ms_out[1] = force pass (EAM: with the
  * fused kick-drift epilogue). Advances the state like run_until. */
 int tdmd_sched_time_passes(tdmd_sched* s, int32_t steps, float* ms_out);
+/* Autotuner (no reference counterpart): times `iters` density + force
+ * passes (force-only epilogue, no change to the trajectory) for each
+ * candidate kernel shape -- neighbour slots per pipelined group {8, 4} x
+ * LDS spline window {device limit, 79 KiB (two workgroups per CU)} -- on
+ * the uploaded state, keeps the fastest and returns it: out[0] =
+ * pair_unroll, out[1] = lds_budget bytes (0 where the potential has no LDS
+ * window to choose). Runs the initial force pass if needed. */
+int tdmd_sched_autotune(tdmd_sched* s, int32_t iters, int32_t out[2]);
 /* Stream the scheduler runs on (for event timing). */
 int tdmd_sched_stream(const tdmd_sched* s, void** stream);
 

@@ -170,6 +170,7 @@ _PROTOS = {
     "tdmd_sched_thermo": (C.c_int, [_VP, _PD]),
     "tdmd_sched_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
     "tdmd_sched_time_passes": (C.c_int, [_VP, _I32, C.POINTER(C.c_float)]),
+    "tdmd_sched_autotune": (C.c_int, [_VP, _I32, _PI32]),
     "tdmd_fm_create": (C.c_int, [C.POINTER(_VP), _I32, C.POINTER(Box), _I32, _I32, _PD, _D]),
     "tdmd_fm_destroy": (C.c_int, [_VP]),
     "tdmd_fm_info": (C.c_int, [_VP, _PI32, C.POINTER(_I64)]),

@@ -77,6 +77,9 @@ struct EngineBase {
   virtual int count_pass(hipStream_t s, int32_t* d_max) = 0;
   virtual int fill_pass(hipStream_t s) = 0;
   virtual int sort_pass(hipStream_t s) = 0;
+  // Time the candidate kernel shapes on the current state and keep the
+  // fastest; out = {pair_unroll, lds_budget bytes (0: not applicable)}.
+  virtual int tune(int iters, int32_t* out) = 0;
 };
 
 }  // namespace tdmd_hip
@@ -138,6 +141,9 @@ struct Engine final : EngineBase {
   // workgroups per CU).
   bool lds = false;
   int unroll = 4;
+  int lds_limit = 0;  // per-workgroup dynamic LDS the kernels are allowed
+  int nint = 0;       // r-grid intervals
+  int budget = 0;     // current window budget
   LdsWindow w_dens[2]{}, w_force[2]{};  // [kEnergy]
   size_t b_dens[2]{}, b_force[2]{};
 
@@ -159,15 +165,28 @@ struct Engine final : EngineBase {
   template <bool kWin, int kU>
   int allow_all() {
     int rc = 0;
-    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin, kU>, b_dens[0])) ||
-        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin, kU>, b_dens[1])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin, kU>, b_force[1])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin, kU>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU>, b_force[0])) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU>, b_force[0])))
+    const size_t L = static_cast<size_t>(lds_limit);
+    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin, kU>, L)) ||
+        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin, kU>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin, kU>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin, kU>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU>, L)))
       return rc;
     return TDMD_OK;
+  }
+
+  // Windows for a budget (bytes, clamped to the allowed limit) and unroll.
+  void configure(int u, int bytes) {
+    budget = std::min(bytes > 0 ? bytes : lds_limit, lds_limit);
+    using V2 = typename Vec2<R>::type;
+    for (int e = 0; e < 2; ++e) {
+      w_dens[e] = window(nint, sizeof(Q), static_cast<size_t>(budget), &b_dens[e]);
+      w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0),
+                          static_cast<size_t>(budget), &b_force[e]);
+    }
+    unroll = u;
   }
 
   int init() override {
@@ -183,23 +202,64 @@ struct Engine final : EngineBase {
     TDMD_HIP_TRY(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     // The dynamic window plus the kernels' static LDS (block reduction)
     // must fit the per-workgroup limit; keep 1 KiB for the static part.
-    int limit = std::min(lmax, 160 * 1024) - 1024;
-    if (S->cfg.lds_budget > 0) limit = std::min(limit, S->cfg.lds_budget);
-    if (limit < 4096) return fail(TDMD_E_INVALID, "scheduler: lds_budget below 4096 bytes");
-    const size_t budget = static_cast<size_t>(limit);
-    const int nint = m[1].n - 1;
-    using V2 = typename Vec2<R>::type;
-    for (int e = 0; e < 2; ++e) {
-      w_dens[e] = window(nint, sizeof(Q), budget, &b_dens[e]);
-      w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0), budget, &b_force[e]);
-    }
-    unroll = S->cfg.pair_unroll == 0 ? 8 : S->cfg.pair_unroll;  // 8: fastest measured
+    lds_limit = std::min(lmax, 160 * 1024) - 1024;
+    if (S->cfg.lds_budget > 0 && S->cfg.lds_budget < 4096)
+      return fail(TDMD_E_INVALID, "scheduler: lds_budget below 4096 bytes");
+    nint = m[1].n - 1;
+    configure(S->cfg.pair_unroll == 0 ? 8 : S->cfg.pair_unroll, S->cfg.lds_budget);
     int rc = 0;
     if ((rc = allow_all<false, 4>()) || (rc = allow_all<true, 4>()) ||
         (rc = allow_all<false, 8>()) || (rc = allow_all<true, 8>()))
       return rc;
     lds = true;
     S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
+    return TDMD_OK;
+  }
+
+  int tune(int iters, int32_t* out) override {
+    out[0] = lds ? unroll : kPairUnroll;
+    out[1] = lds ? budget : 0;
+    if (!lds || S->kind != 1) return TDMD_OK;  // nothing to choose
+    hipStream_t s = S->stream;
+    hipEvent_t e0, e1;
+    TDMD_HIP_TRY(hipEventCreate(&e0));
+    TDMD_HIP_TRY(hipEventCreate(&e1));
+    struct Cand {
+      int u, b;
+    };
+    const Cand cands[] = {{8, lds_limit}, {4, lds_limit}, {8, 80 * 1024 - 1024},
+                          {4, 80 * 1024 - 1024}};
+    float best = 1e30f;
+    Cand pick = cands[0];
+    for (const Cand& c : cands) {
+      configure(c.u, c.b);
+      // density + force with the force-only epilogue: both passes rewrite
+      // M2 and F with the values they already hold, so timing is free of
+      // side effects on the trajectory
+      for (int k = 0; k < 2; ++k) {
+        density<false>(s);
+        force<kForceOnly, false>(s);
+      }
+      TDMD_HIP_TRY(hipEventRecord(e0, s));
+      for (int k = 0; k < iters; ++k) {
+        density<false>(s);
+        force<kForceOnly, false>(s);
+      }
+      TDMD_HIP_TRY(hipEventRecord(e1, s));
+      TDMD_HIP_TRY(hipEventSynchronize(e1));
+      TDMD_LAUNCH_CHECK();
+      float ms = 0.0f;
+      TDMD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        pick = c;
+      }
+    }
+    configure(pick.u, pick.b);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out[0] = unroll;
+    out[1] = budget;
     return TDMD_OK;
   }
 
@@ -898,6 +958,27 @@ int tdmd_sched_create_morse_batch(tdmd_sched** out, int32_t nrep, const tdmd_box
   }
   *out = S;
   return TDMD_OK;
+}
+
+int tdmd_sched_autotune(tdmd_sched* S, int32_t iters, int32_t out[2]) {
+  if (!S || !out || iters < 1) return fail(TDMD_E_INVALID, "sched_autotune: bad arguments");
+  if (!S->uploaded) return fail(TDMD_E_STATE, "sched_autotune: upload first");
+  if (!S->initial_forces) {
+    const int rc = compute_initial_forces(S);
+    if (rc) return rc;
+  }
+  const int rc = S->eng->tune(iters, out);
+  if (rc) return rc;
+  // captured graphs hold the previous kernel choice
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
+  S->graphs.clear();
+  S->graph_defs.clear();
+  S->graph_kernels.clear();
+  S->cfg.pair_unroll = out[0];
+  S->cfg.lds_budget = out[1];
+  return check_flags(S);
 }
 
 int tdmd_sched_destroy(tdmd_sched* s) {

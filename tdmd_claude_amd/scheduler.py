@@ -170,6 +170,13 @@ class FastPipelineScheduler:
         capi.call("tdmd_sched_time_passes", self._h, int(steps), out)
         return float(out[0]), float(out[1])
 
+    def autotune(self, iters: int = 10) -> dict:
+        """Pick the fastest kernel shape on the uploaded state (no change to
+        the trajectory); returns {"pair_unroll", "lds_budget"}."""
+        out = (C.c_int32 * 2)()
+        capi.call("tdmd_sched_autotune", self._h, int(iters), out)
+        return {"pair_unroll": int(out[0]), "lds_budget": int(out[1])}
+
     def stream(self) -> C.c_void_p:
         s = C.c_void_p()
         capi.call("tdmd_sched_stream", self._h, C.byref(s))

@@ -224,3 +224,24 @@ def test_launch_accounting():
     per_step = (b.kernel_launches - a.kernel_launches) / 100.0
     assert 2.0 <= per_step <= 3.2
     assert b.ticks - a.ticks == 100
+
+
+def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged():
+    s = fcc(6)
+    say("autotune")
+    cfg = FastPipelineConfig(precision="mixed")
+    ref = FastPipelineScheduler.for_system(s, potential("johnson"), cfg)
+    ref.run_until(20)
+    p0, v0, _, _, _ = ref.download()
+    tuned = FastPipelineScheduler.for_system(s, potential("johnson"), cfg)
+    tuned.run_until(10)
+    choice = tuned.autotune(3)
+    assert choice["pair_unroll"] in (4, 8) and choice["lds_budget"] > 0
+    tuned.run_until(20)
+    p1, v1, _, _, _ = tuned.download()
+    # same trajectory up to the kernel shape's summation order (bitwise when
+    # the choice equals the default shape)
+    L = s.hi - s.lo
+    d = p1 - p0
+    d -= np.round(d / L) * L
+    assert np.abs(d).max() < 1e-9
