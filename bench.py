@@ -116,8 +116,9 @@ def bench_morse_c2(steps: int) -> dict:
 def bench_force_matching(iters: int) -> dict:
     """Force matching over 256 snapshots of 10^3-cell Cu (4,000 atoms, N(0, 0.05^2)
     displacements, seeds 5..260): loss+adjoint evaluation time for Morse and for the
-    5000-point EAM knot values (mixed), and an Adam fit of `iters` iterations from
-    the unperturbed tables to targets of a perturbed Johnson parameter set."""
+    5000-point EAM knot values (mixed), and Adam fits of `iters` iterations: Morse
+    (D, alpha, r0), and the EAM tables (relative hat basis, 64 nodes per function)
+    from the unperturbed tables to targets of a perturbed Johnson parameter set."""
     import torch
 
     from tdmd_claude_amd import fitting
@@ -159,9 +160,11 @@ def bench_force_matching(iters: int) -> dict:
         fm_e.eam(v0, grid)
     torch.cuda.synchronize()
     eam_ms = 1000 * (time.perf_counter() - t0) / n_eval
+    basis = fitting.RelativeHatBasis(v0, grid, nodes=64)
     t0 = time.perf_counter()
-    _, elosses = fitting.fit(lambda v: fitting.EamForceMatch.apply(v, fm_e, grid), v0,
-                             iters=iters, lr=2e-4)
+    _, elosses = fitting.fit(
+        lambda c: fitting.EamForceMatch.apply(basis.expand(c), fm_e, grid), basis.zeros(),
+        iters=iters, lr=1e-3)
     torch.cuda.synchronize()
     fit_s = time.perf_counter() - t0
     npairs_e = fm_e.npairs()
@@ -176,7 +179,8 @@ def bench_force_matching(iters: int) -> dict:
                   "adam_iters": iters, "loss_start": mlosses[0], "loss_end": mlosses[-1]},
         "eam": {"eval_ms": round(eam_ms, 3), "pairs": npairs_e,
                 "matom_evals_per_s": round(natoms / eam_ms / 1e3, 1),
-                "parameters": grid.nvalues, "adam_iters": iters,
+                "parameters": grid.nvalues, "fit_basis": "relative hat, 64 nodes per function",
+                "adam_iters": iters,
                 "fit_s": round(fit_s, 3), "loss_start": elosses[0], "loss_end": elosses[-1]},
     }
 

@@ -173,6 +173,38 @@ if torch is not None:
             return gout * grad, None, None
 
 
+class RelativeHatBasis:
+    """Smooth relative parametrisation of EAM knot values for fitting:
+    v = v0 * (1 + P c), with P piecewise-linear hat functions on `nodes`
+    evenly spaced nodes per function (F, rho, phi). Adam's per-coordinate
+    steps on raw knots put sign-like noise on neighbouring knots, which the
+    spline derivative amplifies by 1/dr; relative hat coefficients move the
+    tables smoothly and on every magnitude scale at once. The gradient wrt c
+    is the chain rule through P on the knot-value gradient of the adjoint."""
+
+    def __init__(self, values0, grid: EamGrid, nodes: int = 32):
+        self.v0 = values0.detach().clone()
+        dev, dt = values0.device, values0.dtype
+        self.P = [self._hat(grid.nrho, nodes, dev, dt), self._hat(grid.nr, nodes, dev, dt),
+                  self._hat(grid.nr, nodes, dev, dt)]
+        self.nodes = nodes
+
+    @staticmethod
+    def _hat(n, k, dev, dt):
+        x = torch.arange(n, dtype=dt, device=dev)[:, None]
+        nodes = torch.linspace(0, n - 1, k, dtype=dt, device=dev)[None, :]
+        h = (n - 1) / (k - 1)
+        return torch.clamp(1.0 - torch.abs(x - nodes) / h, min=0.0)
+
+    def zeros(self):
+        return torch.zeros(3 * self.nodes, dtype=self.v0.dtype, device=self.v0.device)
+
+    def expand(self, c):
+        cs = torch.split(c, self.nodes)
+        rel = torch.cat([P @ ci for P, ci in zip(self.P, cs)])
+        return self.v0 * (1.0 + rel)
+
+
 def fc_snapshots(cells: int, nsnap: int, sigma: float, seed: int, a: float = 3.615):
     """nsnap FCC configurations with N(0, sigma^2) displacements, seeds
     seed, seed+1, ... -> (System of the first, array (nsnap, N, 3))."""
