@@ -219,6 +219,8 @@ def main() -> int:
                     help="Morse 32k-atom NVE steps per precision (0: skip)")
     ap.add_argument("--fm-iters", type=int, default=200,
                     help="force-matching Adam iterations (0: skip the force-matching leg)")
+    ap.add_argument("--fm-only", action="store_true",
+                    help="run only the force-matching leg and print its JSON (child process)")
     ap.add_argument("--profile", action="store_true",
                     help="short mixed run only (for rocprofv3)")
     args = ap.parse_args()
@@ -230,6 +232,13 @@ def main() -> int:
     if world != 1:
         print("bench.py is single-process; WORLD_SIZE must be 1", file=sys.stderr)
         return 2
+
+    if args.fm_only:
+        # torch first: it carries its own HIP runtime, and the library then
+        # binds to the same one in this process
+        import torch  # noqa: F401
+        print(json.dumps(bench_force_matching(args.fm_iters)), flush=True)
+        return 0
 
     capi.call("tdmd_set_device", 0)
     name = capi.C.create_string_buffer(256)
@@ -309,7 +318,16 @@ def main() -> int:
         log(f"Morse 32k: mixed {extras['morse_32k']['mixed']['value']} / fp64 "
             f"{extras['morse_32k']['fp64']['value']} Matom-steps/s")
     if args.fm_iters > 0:
-        extras["force_matching"] = bench_force_matching(args.fm_iters)
+        # own process: torch brings its own HIP runtime, this process already
+        # runs on the system one
+        import subprocess
+
+        res = subprocess.run([sys.executable, os.path.abspath(__file__), "--fm-only",
+                              "--fm-iters", str(args.fm_iters)],
+                             capture_output=True, text=True, timeout=600)
+        if res.returncode != 0:
+            raise RuntimeError(f"force-matching leg failed:\n{res.stderr[-2000:]}")
+        extras["force_matching"] = json.loads(res.stdout.strip().splitlines()[-1])
         fmx = extras["force_matching"]
         log(f"force matching: Morse eval {fmx['morse']['eval_ms']} ms, EAM eval "
             f"{fmx['eam']['eval_ms']} ms, EAM fit {fmx['eam']['fit_s']} s "
