@@ -113,6 +113,37 @@ def bench_morse_c2(steps: int) -> dict:
     return out
 
 
+def bench_replicas(steps: int) -> dict:
+    """64 replicas of random equiatomic Ni/Co/Cr FCC alloys (a = 3.57 A, sizes
+    log-uniform in [500, 32000], T ~ U(300, 1200) K, master seed 2024),
+    three-element Johnson-form EAM (5000-point tables, cutoff 5.0 A), NVE,
+    dt 1 fs, all replicas advanced together (one launch per pass), shape
+    chosen by the autotuner: mixed and FP64 throughput."""
+    from tdmd_claude_amd.scheduler import BatchScheduler
+
+    systems, temps = I.alloy_replicas(64, 2024)
+    tables = I.johnson_alloy_setfl()
+    natoms = sum(s.natoms for s in systems)
+    out = {"workload": "64 Ni/Co/Cr FCC replicas, sizes log-uniform in [500, 32000] atoms, "
+                       "3-element EAM (5000-point tables, rc 5.0 A), NVE, dt 1 fs, skin 1 A "
+                       "displacement rebuild", "natoms": natoms, "replicas": len(systems),
+           "smallest": min(s.natoms for s in systems), "largest": max(s.natoms for s in systems),
+           "steps": steps}
+    for prec in ("mixed", "fp64"):
+        cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=prec,
+                                 rebuild_policy="displacement")
+        b = BatchScheduler.for_systems(systems, tables, cfg)
+        b.run_until(0)
+        tuned = b.autotune(5)
+        wall, _ = timed_run(b, 50, steps)
+        out[prec] = {"value": round(natoms * steps / wall / 1e6, 2),
+                     "ms_per_step": round(1000 * wall / steps, 4),
+                     "rebuilds": b.stats().rebuilds, "autotune": tuned}
+        b.close()
+    out["mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
 def bench_force_matching(iters: int) -> dict:
     """Force matching over 256 snapshots of 10^3-cell Cu (4,000 atoms, N(0, 0.05^2)
     displacements, seeds 5..260): loss+adjoint evaluation time for Morse and for the
@@ -185,6 +216,22 @@ def bench_force_matching(iters: int) -> dict:
     }
 
 
+def measured_traffic(natoms: int, mirror_fp64: bool):
+    """HBM bytes per launch of the force pass from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/eam_passes_1M_mixed_traffic.json),
+    when they were taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "eam_passes_1M_mixed_traffic.json")
+    if mirror_fp64 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        doc = json.load(fh)
+    if doc.get("natoms") != natoms:
+        return None
+    k = doc["kernels"]["force_nve"]
+    return {"bytes": round(k["hbm_bytes"]), "bytes_per_atom": round(k["bytes_per_atom"], 1),
+            "source": "profiles/eam_passes_1M_mixed_traffic.json"}
+
+
 def nve_drift(system: I.System, tables, precision: str, steps: int, every: int) -> dict:
     """Mixed NVE run from the given state: slope of total energy per atom."""
     cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=precision,
@@ -217,6 +264,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=10000,
                     help="Morse 32k-atom NVE steps per precision (0: skip)")
+    ap.add_argument("--replica-steps", type=int, default=1000,
+                    help="64-replica alloy steps per precision (0: skip)")
     ap.add_argument("--fm-iters", type=int, default=200,
                     help="force-matching Adam iterations (0: skip the force-matching leg)")
     ap.add_argument("--fm-only", action="store_true",
@@ -292,6 +341,7 @@ def main() -> int:
     t_dens, t_force = nve.time_passes(30)
     nve.close()
     bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar)
+    traffic = measured_traffic(N, args.mirror_fp64)
     achieved = N * bpa["force"] / (t_force * 1e-3) / 1e9
     log(f"passes: density {t_dens:.4f} ms, force {t_force:.4f} ms; force pass "
         f"{achieved:.0f} GB/s algorithmic ({bpa['force']:.0f} B/atom)")
@@ -317,6 +367,11 @@ def main() -> int:
         extras["morse_32k"] = bench_morse_c2(args.c2_steps)
         log(f"Morse 32k: mixed {extras['morse_32k']['mixed']['value']} / fp64 "
             f"{extras['morse_32k']['fp64']['value']} Matom-steps/s")
+    if args.replica_steps > 0:
+        extras["replicas_64"] = bench_replicas(args.replica_steps)
+        log(f"64 replicas ({extras['replicas_64']['natoms']} atoms): mixed "
+            f"{extras['replicas_64']['mixed']['value']} / fp64 "
+            f"{extras['replicas_64']['fp64']['value']} Matom-steps/s")
     if args.fm_iters > 0:
         # own process: torch brings its own HIP runtime, this process already
         # runs on the system one
@@ -391,12 +446,13 @@ def main() -> int:
         "nve_energy_drift": drift,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_eam_force (force + fused kick-drift)",
+            "kernel": "k_eam_force_lds<float, MirrorFixed> (force + fused NVE kick-drift)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else traffic["bytes"],
+            "traffic_per_atom": None if traffic is None else traffic["bytes_per_atom"],
             "bytes_per_atom": round(bpa["force"], 1),
             "avg_ms": round(t_force, 5),
             "density_pass_ms": round(t_dens, 5),

@@ -26,6 +26,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "sched_kernels.inl"
 #include "sched_lds.inl"
 
@@ -80,6 +82,13 @@ struct EngineBase {
   // Time the candidate kernel shapes on the current state and keep the
   // fastest; out = {pair_unroll, lds_budget bytes (0: not applicable)}.
   virtual int tune(int iters, int32_t* out) = 0;
+  // Bracket the next force-path launch (EAM density / force, Morse
+  // kick-drift / force) with events stamped by the dispatch itself.
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  void time_next(hipEvent_t a, hipEvent_t b) {
+    ev_start = a;
+    ev_stop = b;
+  }
 };
 
 }  // namespace tdmd_hip
@@ -128,6 +137,18 @@ struct Engine final : EngineBase {
   explicit Engine(tdmd_sched* s) : S(s) {}
 
   using Q = typename Quad<R>::type;
+
+  template <class K, class... Args>
+  void go(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
+    if (ev_start) {
+      hipExtLaunchKernelGGL(kernel, grid, block, static_cast<uint32_t>(lds), s, ev_start, ev_stop,
+                            0u, args...);
+      ev_start = ev_stop = nullptr;
+    } else {
+      kernel<<<grid, block, lds, s>>>(args...);
+    }
+  }
+
   const SplineMeta* meta() const { return static_cast<const SplineMeta*>(S->tables->d_meta); }
   const Q* coeff() const { return static_cast<const Q*>(S->tables->d_coeff); }
   double rc_sq() const { return S->tables->cutoff * S->tables->cutoff; }
@@ -266,13 +287,14 @@ struct Engine final : EngineBase {
   template <bool kEnergy, bool kWin, int kU>
   void density_lds(hipStream_t s) {
     const int nb = S->nb_force;
-    k_eam_density_lds<R, Mir, kEnergy, kWin, kU><<<nb, kBigBlock, b_dens[kEnergy], s>>>(
-        S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
+    go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU>, nb, kBigBlock, b_dens[kEnergy], s, S->a,
+       meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
   }
   template <bool kEnergy>
   void density(hipStream_t s) {
     if (!lds) {
-      k_eam_density<R, Mir, kEnergy><<<S->nblocks, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+      go(k_eam_density<R, Mir, kEnergy>, S->nblocks, kBlock, 0, s, S->a, meta(), coeff(),
+         rc_sq());
       return;
     }
     const bool win = w_dens[kEnergy].k_lo > 0;
@@ -285,8 +307,8 @@ struct Engine final : EngineBase {
   template <int kEpi, bool kEnergy, bool kWin, int kU>
   void force_lds(hipStream_t s) {
     const int nb = S->nb_force;
-    k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU><<<nb, kBigBlock, b_force[kEnergy], s>>>(
-        S->a, meta(), coeff(), rc_sq(), w_force[kEnergy], nb);
+    go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU>, nb, kBigBlock, b_force[kEnergy], s, S->a,
+       meta(), coeff(), rc_sq(), w_force[kEnergy], nb);
   }
 
   template <int kEpi, bool kEnergy>
@@ -299,12 +321,14 @@ struct Engine final : EngineBase {
       else if (lds)
         win ? force_lds<kEpi, kEnergy, true, 4>(s) : force_lds<kEpi, kEnergy, false, 4>(s);
       else if (S->tables->ntypes > 1)
-        k_eam_force<R, Mir, kEpi, kEnergy, true><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+        go(k_eam_force<R, Mir, kEpi, kEnergy, true>, nb, kBlock, 0, s, S->a, meta(), coeff(),
+           rc_sq());
       else
-        k_eam_force<R, Mir, kEpi, kEnergy, false><<<nb, kBlock, 0, s>>>(S->a, meta(), coeff(), rc_sq());
+        go(k_eam_force<R, Mir, kEpi, kEnergy, false>, nb, kBlock, 0, s, S->a, meta(), coeff(),
+           rc_sq());
     } else {
       if constexpr (kEpi != kFullKickDrift)
-        k_morse_force<R, Mir, kEpi, kEnergy><<<nb, kBlock, 0, s>>>(S->a, mp());
+        go(k_morse_force<R, Mir, kEpi, kEnergy>, nb, kBlock, 0, s, S->a, mp());
     }
   }
 
@@ -369,7 +393,7 @@ struct Engine final : EngineBase {
         k_kick_drift<R, Mir, false, false><<<nb, kBlock, 0, s>>>(S->a);
         break;
       case OP_KD_FULL:
-        k_kick_drift<R, Mir, false, true><<<nb, kBlock, 0, s>>>(S->a);
+        go(k_kick_drift<R, Mir, false, true>, nb, kBlock, 0, s, S->a);
         break;
       case OP_KD_NVT:
         k_kick_drift<R, Mir, true, false><<<nb, kBlock, 0, s>>>(S->a);
@@ -1136,17 +1160,19 @@ int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
   const bool eam = S->kind == 1;
   const char a_op = eam ? OP_DENS : OP_KD_FULL;
   const char b_op = eam ? OP_F_FULL : OP_F_ONLY;
-  std::vector<hipEvent_t> ev(static_cast<size_t>(3 * steps));
+  // Events are stamped by the kernel dispatches themselves
+  // (hipExtLaunchKernel), so each interval is one kernel's execution.
+  std::vector<hipEvent_t> ev(static_cast<size_t>(4 * steps));
   for (auto& e : ev) TDMD_HIP_TRY(hipEventCreate(&e));
   if ((rc = S->eng->launch(OP_BOOT, s))) return rc;
   if (!eam && (rc = S->eng->launch(OP_F_ONLY, s))) return rc;
   for (int k = 0; k < steps - 1; ++k) {
-    TDMD_HIP_TRY(hipEventRecord(ev[3 * k], s));
+    S->eng->time_next(ev[4 * k], ev[4 * k + 1]);
     if ((rc = S->eng->launch(a_op, s))) return rc;
-    TDMD_HIP_TRY(hipEventRecord(ev[3 * k + 1], s));
+    S->eng->time_next(ev[4 * k + 2], ev[4 * k + 3]);
     if ((rc = S->eng->launch(b_op, s))) return rc;
-    TDMD_HIP_TRY(hipEventRecord(ev[3 * k + 2], s));
   }
+  S->eng->time_next(nullptr, nullptr);
   if (eam && (rc = S->eng->launch(OP_DENS, s))) return rc;
   if (!eam && (rc = S->eng->launch(OP_KD_FULL, s))) return rc;
   if ((rc = S->eng->launch(OP_F_HALF, s))) return rc;
@@ -1154,8 +1180,8 @@ int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
   double ta = 0, tb = 0;
   for (int k = 0; k < steps - 1; ++k) {
     float x, y;
-    TDMD_HIP_TRY(hipEventElapsedTime(&x, ev[3 * k], ev[3 * k + 1]));
-    TDMD_HIP_TRY(hipEventElapsedTime(&y, ev[3 * k + 1], ev[3 * k + 2]));
+    TDMD_HIP_TRY(hipEventElapsedTime(&x, ev[4 * k], ev[4 * k + 1]));
+    TDMD_HIP_TRY(hipEventElapsedTime(&y, ev[4 * k + 2], ev[4 * k + 3]));
     ta += x;
     tb += y;
   }
