@@ -39,6 +39,7 @@ struct FastPipelineConfig {
   i32 max_neighbors{0};
   i32 pair_unroll{0};
   i32 lds_budget{0};
+  bool split_rebuild{false};
 };
 
 struct FastPipelineStats {
@@ -153,6 +154,7 @@ class FastPipelineScheduler {
     c.max_neighbors = cfg.max_neighbors;
     c.pair_unroll = cfg.pair_unroll;
     c.lds_budget = cfg.lds_budget;
+    c.split_rebuild = cfg.split_rebuild ? 1 : 0;
     return c;
   }
 

@@ -271,7 +271,9 @@ typedef struct {
                           /* 4 or 8 (single-element EAM passes)           */
   int32_t lds_budget;     /* LDS bytes per workgroup for spline windows:  */
                           /* 0 = device limit                             */
-  int32_t reserved[6];
+  int32_t split_rebuild;  /* 1: neighbour rebuild as separate kernels;    */
+                          /* 0 (default): one fused launch per step       */
+  int32_t reserved[5];
 } tdmd_sched_config;
 
 void tdmd_sched_config_default(tdmd_sched_config* cfg);

@@ -83,7 +83,8 @@ class SchedConfig(C.Structure):
         ("max_neighbors", C.c_int32),
         ("pair_unroll", C.c_int32),
         ("lds_budget", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("split_rebuild", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

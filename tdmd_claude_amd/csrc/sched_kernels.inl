@@ -416,7 +416,6 @@ struct GridArgs {
   int32_t* cstart;         // [ncells] by rank
   int32_t* cursor;         // [ncells] by rank
   int32_t* key;            // [N] rank of each atom's cell (old order)
-  int32_t* skey;           // [N] rank of each atom's cell (new order)
   int32_t* sidx;           // [N] new slot -> old index
   int32_t* orig;           // [N] upload index
   int32_t* done;           // completion counter
@@ -482,11 +481,11 @@ __device__ __forceinline__ void cell_coords(const CellGrid& cg, double x, double
   }
 }
 
+// ---- per-item bodies, shared by the separate kernels (first build) and
+// the fused rebuild kernel ----
+
 template <class Mir>
-__global__ void __launch_bounds__(kBlock) k_bin(StepArgs a, GridArgs g) {
-  if (!a.flags[0]) return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.natoms) return;
+__device__ __forceinline__ void bin_atom(const StepArgs& a, const GridArgs& g, int i) {
   int c[3];
   const CellGrid cg = grid_of<Mir>(a, g, i);
   cell_coords(cg, a.px[i], a.py[i], a.pz[i], c);
@@ -495,15 +494,16 @@ __global__ void __launch_bounds__(kBlock) k_bin(StepArgs a, GridArgs g) {
   atomicAdd(&g.ccount[rank], 1);
 }
 
-// Single-block exclusive scan of the cell counts (atomics-free).
-__global__ void __launch_bounds__(1024) k_cell_scan(const int32_t* flags, GridArgs g) {
-  if (!flags[0]) return;
-  __shared__ int wsum[16];
+// Exclusive scan of the cell counts by one workgroup of BLOCK threads
+// (atomics-free).
+template <int BLOCK>
+__device__ __forceinline__ void scan_cells(const GridArgs& g) {
+  __shared__ int wsum[BLOCK / 64];
   __shared__ int carry_s;
   if (threadIdx.x == 0) carry_s = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int start = 0; start < g.ncells; start += 1024) {
+  for (int start = 0; start < g.ncells; start += BLOCK) {
     const int c = start + static_cast<int>(threadIdx.x);
     const int v = c < g.ncells ? g.ccount[c] : 0;
     int incl = v;
@@ -519,26 +519,20 @@ __global__ void __launch_bounds__(1024) k_cell_scan(const int32_t* flags, GridAr
     __syncthreads();
     if (threadIdx.x == 0) {
       int t = 0;
-      for (int k = 0; k < 16; ++k) t += wsum[k];
+      for (int k = 0; k < BLOCK / 64; ++k) t += wsum[k];
       carry_s += t;
     }
     __syncthreads();
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_scatter(StepArgs a, GridArgs g) {
-  if (!a.flags[0]) return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.natoms) return;
+__device__ __forceinline__ void scatter_atom(const GridArgs& g, int i) {
   const int r = g.key[i];
   g.sidx[g.cstart[r] + atomicAdd(&g.cursor[r], 1)] = i;
 }
 
 // Deterministic order inside a cell: by upload index.
-__global__ void __launch_bounds__(kBlock) k_order(StepArgs a, GridArgs g) {
-  if (!a.flags[0]) return;
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= g.ncells) return;
+__device__ __forceinline__ void order_cell(const GridArgs& g, int r) {
   int32_t* seg = g.sidx + g.cstart[r];
   const int n = g.ccount[r];
   for (int x = 1; x < n; ++x) {
@@ -554,10 +548,8 @@ __global__ void __launch_bounds__(kBlock) k_order(StepArgs a, GridArgs g) {
 }
 
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_permute(StepArgs a, GridArgs g, PermArgs t) {
-  if (!a.flags[0]) return;
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= a.natoms) return;
+__device__ __forceinline__ void permute_atom(const StepArgs& a, const GridArgs& g,
+                                             const PermArgs& t, int s) {
   const int o = g.sidx[s];
   t.px[s] = a.px[o];
   t.py[s] = a.py[o];
@@ -570,14 +562,11 @@ __global__ void __launch_bounds__(kBlock) k_permute(StepArgs a, GridArgs g, Perm
   rp<R>(t.fz)[s] = rp<R>(a.fz)[o];
   t.types[s] = a.types[o];
   t.orig[s] = g.orig[o];
-  g.skey[s] = g.key[o];
 }
 
 template <class R, class Mir>
-__global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, PermArgs t) {
-  if (!a.flags[0]) return;
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= a.natoms) return;
+__device__ __forceinline__ void copyback_atom(const StepArgs& a, const GridArgs& g,
+                                              const PermArgs& t, int s) {
   const double x = t.px[s], y = t.py[s], z = t.pz[s];
   a.px[s] = x;
   a.py[s] = y;
@@ -597,9 +586,9 @@ __global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, Per
   static_cast<typename Mir::T*>(a.M)[s] = Mir::encode(mp_of<Mir>(a, s), x, y, z, ty - 1);
 }
 
-// ELL list over the sorted atoms. Inclusion exactly as the CPU list
-// (neighbor_list.cpp:58-62): d = x_j - x_i in FP64, single minimum image,
-// unfused r2 < r_list^2. kCount: only count (capacity sizing).
+// ELL row of atom i over the sorted atoms; returns the true count. Inclusion
+// exactly as the CPU list (neighbor_list.cpp:58-62): d = x_j - x_i in FP64,
+// single minimum image, unfused r2 < r_list^2. kCount: only count.
 //
 // Each row is partitioned: neighbours inside the interaction cutoff at build
 // time first, the skin shell (rc <= r < r_list) after them. Atoms move less
@@ -609,70 +598,130 @@ __global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, Per
 // shell. Near entries are written from the front and shell entries from the
 // back of the row; the shell block is then moved down behind the near block.
 template <bool kCount, class Mir>
-__global__ void __launch_bounds__(kBlock)
-k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, double near_sq, int32_t* nbr,
-            int32_t* cnt, int32_t* maxcnt) {
-  if (!kCount && !a.flags[0]) return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, int i,
+                                          double r_list_sq, double near_sq, int32_t* nbr,
+                                          int32_t* cnt) {
   int count = 0, nn = 0, nf = 0;
   const int cap = a.cap;
-  if (i < a.natoms) {
-    const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
-    const MirrorParams& mp = mp_of<Mir>(a, i);
-    const CellGrid cg = grid_of<Mir>(a, g, i);
-    int c[3];
-    cell_coords(cg, xi, yi, zi, c);
-    for (int dz = -1; dz <= 1; ++dz) {
-      int nz = c[2] + dz;
-      nz = nz < 0 ? nz + cg.n[2] : (nz >= cg.n[2] ? nz - cg.n[2] : nz);
-      for (int dy = -1; dy <= 1; ++dy) {
-        int ny = c[1] + dy;
-        ny = ny < 0 ? ny + cg.n[1] : (ny >= cg.n[1] ? ny - cg.n[1] : ny);
-        for (int dx = -1; dx <= 1; ++dx) {
-          int nx = c[0] + dx;
-          nx = nx < 0 ? nx + cg.n[0] : (nx >= cg.n[0] ? nx - cg.n[0] : nx);
-          const int r = g.rank_of[cg.off + (nz * cg.n[1] + ny) * cg.n[0] + nx];
-          const int start = g.cstart[r], end = start + g.ccount[r];
-          for (int j = start; j < end; ++j) {
-            if (j == i) continue;
-            const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
-            const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
-            const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
-            const double r2 = r2_unfused(ddx, ddy, ddz);
-            if (r2 < r_list_sq) {
-              if (!kCount && nn + nf < cap) {
-                if (r2 < near_sq)
-                  nbr[ell(i, nn++, cap)] = j;
-                else
-                  nbr[ell(i, cap - 1 - nf++, cap)] = j;
-              }
-              ++count;
+  const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
+  const MirrorParams& mp = mp_of<Mir>(a, i);
+  const CellGrid cg = grid_of<Mir>(a, g, i);
+  int c[3];
+  cell_coords(cg, xi, yi, zi, c);
+  for (int dz = -1; dz <= 1; ++dz) {
+    int nz = c[2] + dz;
+    nz = nz < 0 ? nz + cg.n[2] : (nz >= cg.n[2] ? nz - cg.n[2] : nz);
+    for (int dy = -1; dy <= 1; ++dy) {
+      int ny = c[1] + dy;
+      ny = ny < 0 ? ny + cg.n[1] : (ny >= cg.n[1] ? ny - cg.n[1] : ny);
+      for (int dx = -1; dx <= 1; ++dx) {
+        int nx = c[0] + dx;
+        nx = nx < 0 ? nx + cg.n[0] : (nx >= cg.n[0] ? nx - cg.n[0] : nx);
+        const int r = g.rank_of[cg.off + (nz * cg.n[1] + ny) * cg.n[0] + nx];
+        const int start = g.cstart[r], end = start + g.ccount[r];
+        for (int j = start; j < end; ++j) {
+          if (j == i) continue;
+          const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
+          const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
+          const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
+          const double r2 = r2_unfused(ddx, ddy, ddz);
+          if (r2 < r_list_sq) {
+            if (!kCount && nn + nf < cap) {
+              if (r2 < near_sq)
+                nbr[ell(i, nn++, cap)] = j;
+              else
+                nbr[ell(i, cap - 1 - nf++, cap)] = j;
             }
+            ++count;
           }
         }
       }
     }
-    if (!kCount) {
-      // Shell entries sit at [cap - nf, cap); the gap [nn, cap - nf) has
-      // room for the m that must move (the rest already lie in [nn, nn+nf)).
-      const int gap = cap - nn - nf;
-      const int m = nf < gap ? nf : gap;
-      for (int q = 0; q < m; ++q) nbr[ell(i, nn + q, cap)] = nbr[ell(i, cap - 1 - q, cap)];
-      const int n = nn + nf;  // == min(count, cap)
-      cnt[i] = n;
-      // Pad the row to a multiple of 8 with the atom itself, so unrolled
-      // readers can load whole groups of slots (cap is a multiple of 8).
-      for (int k = n; k < ((n + 7) & ~7); ++k) nbr[ell(i, k, cap)] = i;
-    }
   }
+  if (!kCount) {
+    // Shell entries sit at [cap - nf, cap); the gap [nn, cap - nf) has
+    // room for the m that must move (the rest already lie in [nn, nn+nf)).
+    const int gap = cap - nn - nf;
+    const int m = nf < gap ? nf : gap;
+    for (int q = 0; q < m; ++q) nbr[ell(i, nn + q, cap)] = nbr[ell(i, cap - 1 - q, cap)];
+    const int n = nn + nf;  // == min(count, cap)
+    cnt[i] = n;
+    // Pad the row to a multiple of 8 with the atom itself, so unrolled
+    // readers can load whole groups of slots (cap is a multiple of 8).
+    for (int k = n; k < ((n + 7) & ~7); ++k) nbr[ell(i, k, cap)] = i;
+  }
+  return count;
+}
+
+// Largest row count of the wave: recorded for sizing (kCount) or flagged as
+// an overflow of the capacity.
+template <bool kCount>
+__device__ __forceinline__ void report_count(const StepArgs& a, int count, int32_t* maxcnt) {
   const int m = wave_max_i(count);
   if ((threadIdx.x & 63) == 0 && m > 0) {
     if (kCount) {
       atomicMax(maxcnt, m);
-    } else if (m > cap) {
+    } else if (m > a.cap) {
       atomicMax(&a.flags[1], m);
     }
   }
+}
+
+// ---- the separate kernels (first build; every one returns at once when
+// the rebuild flag is clear) ----
+
+template <class Mir>
+__global__ void __launch_bounds__(kBlock) k_bin(StepArgs a, GridArgs g) {
+  if (!a.flags[0]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  bin_atom<Mir>(a, g, i);
+}
+
+__global__ void __launch_bounds__(1024) k_cell_scan(const int32_t* flags, GridArgs g) {
+  if (!flags[0]) return;
+  scan_cells<1024>(g);
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(StepArgs a, GridArgs g) {
+  if (!a.flags[0]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  scatter_atom(g, i);
+}
+
+__global__ void __launch_bounds__(kBlock) k_order(StepArgs a, GridArgs g) {
+  if (!a.flags[0]) return;
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= g.ncells) return;
+  order_cell(g, r);
+}
+
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_permute(StepArgs a, GridArgs g, PermArgs t) {
+  if (!a.flags[0]) return;
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= a.natoms) return;
+  permute_atom<R>(a, g, t, s);
+}
+
+template <class R, class Mir>
+__global__ void __launch_bounds__(kBlock) k_copyback(StepArgs a, GridArgs g, PermArgs t) {
+  if (!a.flags[0]) return;
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= a.natoms) return;
+  copyback_atom<R, Mir>(a, g, t, s);
+}
+
+template <bool kCount, class Mir>
+__global__ void __launch_bounds__(kBlock)
+k_nlist_ell(StepArgs a, GridArgs g, double r_list_sq, double near_sq, int32_t* nbr, int32_t* cnt,
+            int32_t* maxcnt) {
+  if (!kCount && !a.flags[0]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  int count = 0;
+  if (i < a.natoms) count = nlist_atom<kCount, Mir>(a, g, i, r_list_sq, near_sq, nbr, cnt);
+  report_count<kCount>(a, count, maxcnt);
 }
 
 // End of a rebuild: the last block to finish clears the flag (every block has
@@ -689,6 +738,95 @@ __global__ void __launch_bounds__(kBlock) k_rebuild_done(int32_t* flags, int32_t
       flags[0] = 0;
       *rebuilds += 1;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused rebuild: the whole sort + list sequence in one launch, its phases
+// separated by a grid barrier, so a step that needs no rebuild costs one
+// early-returning launch instead of nine.
+//
+// The grid is sized by the host to what the device holds resident at once
+// (occupancy x CUs), so every workgroup of the launch runs concurrently and
+// the barrier can complete. The barrier is a count + generation pair in
+// device memory; its wait is bounded: a wait longer than kBarrierSpins
+// polls sets flags[3] and the kernel returns (the host reports the error)
+// instead of hanging. Writes of one phase are published to the other XCDs'
+// caches by agent-scope fences on both sides of the barrier.
+// ---------------------------------------------------------------------------
+
+struct GridBarrier {
+  unsigned int count;
+  unsigned int gen;
+};
+
+constexpr long kBarrierSpins = 1L << 22;
+
+__device__ __forceinline__ bool grid_barrier(GridBarrier* b, int32_t* flags) {
+  __shared__ int ok;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    const unsigned int gen =
+        __hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int arrived =
+        __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&b->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > kBarrierSpins) {
+          ok = 0;
+          atomicExch(&flags[3], 1);
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  __threadfence();
+  return ok != 0;
+}
+
+template <class R, class Mir>
+__global__ void __launch_bounds__(kBlock)
+k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double near_sq,
+                int32_t* nbr, int32_t* cnt, GridBarrier* bar, int64_t* rebuilds) {
+  if (!a.flags[0]) return;  // uniform: nothing writes the flag before the last phase
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  const int nthr = gridDim.x * kBlock;
+  const int N = a.natoms;
+  for (int r = tid; r < g.ncells; r += nthr) {
+    g.ccount[r] = 0;
+    g.cursor[r] = 0;
+  }
+  if (!grid_barrier(bar, a.flags)) return;
+  for (int i = tid; i < N; i += nthr) bin_atom<Mir>(a, g, i);
+  if (!grid_barrier(bar, a.flags)) return;
+  if (blockIdx.x == 0) scan_cells<kBlock>(g);
+  if (!grid_barrier(bar, a.flags)) return;
+  for (int i = tid; i < N; i += nthr) scatter_atom(g, i);
+  if (!grid_barrier(bar, a.flags)) return;
+  for (int r = tid; r < g.ncells; r += nthr) order_cell(g, r);
+  if (!grid_barrier(bar, a.flags)) return;
+  for (int s = tid; s < N; s += nthr) permute_atom<R>(a, g, t, s);
+  if (!grid_barrier(bar, a.flags)) return;
+  for (int s = tid; s < N; s += nthr) copyback_atom<R, Mir>(a, g, t, s);
+  if (!grid_barrier(bar, a.flags)) return;
+  int count = 0;
+  for (int i = tid; i < N; i += nthr) {
+    const int c = nlist_atom<false, Mir>(a, g, i, r_list_sq, near_sq, nbr, cnt);
+    count = c > count ? c : count;
+  }
+  report_count<false>(a, count, nullptr);
+  // every thread read the flag at entry, before the first barrier
+  if (tid == 0) {
+    a.flags[0] = 0;
+    *rebuilds += 1;
   }
 }
 

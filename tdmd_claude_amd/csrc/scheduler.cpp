@@ -99,6 +99,8 @@ struct tdmd_sched {
   int natoms = 0;
   int nblocks = 0;   // 256-thread grid (multiple of 8)
   int nb_force = 0;  // grid of the EAM/Morse passes = block partials they emit
+  int nb_rebuild = 0;  // fused rebuild grid (all resident at once); 0: separate kernels
+  tdmd_hip::sched::GridBarrier* d_bar = nullptr;
   tdmd_sched_config cfg{};
   tdmd_morse_params morse{};
   tdmd_eam_tables* tables = nullptr;
@@ -212,6 +214,19 @@ struct Engine final : EngineBase {
 
   int init() override {
     S->nb_force = S->nblocks;
+    S->nb_rebuild = 0;
+    if (!S->cfg.split_rebuild) {
+      // every workgroup of the fused rebuild must be resident at once
+      int dev = 0, ncu = 0, per_cu = 0;
+      TDMD_HIP_TRY(hipGetDevice(&dev));
+      TDMD_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      TDMD_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(&k_rebuild_fused<R, Mir>), kBlock, 0));
+      // at most two workgroups per CU: a margin under the occupancy limit
+      const int need = std::max(grid_for(S->natoms), 1);
+      S->nb_rebuild = std::max(1, std::min(std::min(per_cu, 2) * ncu, need));
+      if (per_cu < 1) S->nb_rebuild = 0;  // cannot guarantee residency: separate kernels
+    }
     if (S->kind != 1 || S->tables->ntypes != 1) return TDMD_OK;
     // rho_0 and phi_00 must share the r grid (setfl: one Nr, dr).
     SplineMeta m[3];
@@ -332,6 +347,21 @@ struct Engine final : EngineBase {
     }
   }
 
+  // Conditional rebuild: one fused launch, or the separate kernels.
+  int rebuild(hipStream_t s) {
+    if (S->nb_rebuild > 0) {
+      k_rebuild_fused<R, Mir><<<S->nb_rebuild, kBlock, 0, s>>>(
+          S->a, S->g, S->t, S->r_list * S->r_list, S->cutoff * S->cutoff,
+          const_cast<int32_t*>(S->a.nbr), const_cast<int32_t*>(S->a.cnt), S->d_bar,
+          S->d_rebuilds);
+      ++S->launches;
+      TDMD_LAUNCH_CHECK();
+      return TDMD_OK;
+    }
+    const int rc = sort_pass(s);
+    return rc ? rc : fill_pass(s);
+  }
+
   int sort_pass(hipStream_t s) override {
     const int gb = grid_for(S->natoms), gc = grid_for(S->g.ncells);
     k_zero_cells<<<gc, kBlock, 0, s>>>(S->a.flags, S->g);
@@ -415,13 +445,10 @@ struct Engine final : EngineBase {
       case OP_REB_FORCED: {
         k_set_flag<<<1, 64, 0, s>>>(S->a.flags);
         ++S->launches;
-        const int rc = sort_pass(s);
-        return rc ? rc : fill_pass(s);
+        return rebuild(s);
       }
-      case OP_REB_COND: {
-        const int rc = sort_pass(s);
-        return rc ? rc : fill_pass(s);
-      }
+      case OP_REB_COND:
+        return rebuild(s);
       case OP_DENS:
         density<false>(s);
         break;
@@ -510,13 +537,14 @@ int sched_alloc(tdmd_sched* S) {
   if ((rc = dalloc(S, &g.cstart, g.ncells))) return rc;
   if ((rc = dalloc(S, &g.cursor, g.ncells))) return rc;
   if ((rc = dalloc(S, &g.key, N))) return rc;
-  if ((rc = dalloc(S, &g.skey, N))) return rc;
   if ((rc = dalloc(S, &g.sidx, N))) return rc;
   if ((rc = dalloc(S, &g.orig, N))) return rc;
   if ((rc = dalloc(S, &g.done, 4))) return rc;
   if ((rc = dalloc(S, &S->d_maxcnt, 4))) return rc;
   if ((rc = dalloc(S, &S->d_rebuilds, 1))) return rc;
   if ((rc = dalloc(S, &S->d_scalars, 16))) return rc;
+  if ((rc = dalloc(S, &S->d_bar, 1))) return rc;
+  TDMD_HIP_TRY(hipMemset(S->d_bar, 0, sizeof(GridBarrier)));
   // Morton rank of every cell of each box's grid; a box's cells take the
   // global ranks [cell_off, cell_off + ncells_box), so the sort keeps every
   // box in its own slot range.
@@ -694,6 +722,9 @@ int check_flags(tdmd_sched* S) {
     return fail(TDMD_E_NONFINITE,
                 "scheduler: non-finite position, or an atom outside the representable range "
                 "of an open axis (lo - 2W .. lo + 2W, W = max(box length, r_list))");
+  if (f[3])
+    return fail(TDMD_E_HIP, "scheduler: fused rebuild grid barrier timed out (workgroups not "
+                            "co-resident); use split_rebuild");
   if (f[1] > S->a.cap)
     return fail(TDMD_E_OVERFLOW, "scheduler: neighbour capacity " + std::to_string(S->a.cap) +
                                      " exceeded (max count " + std::to_string(f[1]) + ")");

@@ -41,6 +41,7 @@ class FastPipelineConfig:
     max_neighbors: int = 0
     pair_unroll: int = 0  # neighbour slots per pipelined group (0 = default)
     lds_budget: int = 0  # LDS bytes per workgroup for spline windows (0 = device limit)
+    split_rebuild: bool = False  # rebuild as separate kernels instead of one fused launch
 
     def to_c(self) -> capi.SchedConfig:
         c = capi.SchedConfig()
@@ -59,6 +60,7 @@ class FastPipelineConfig:
         c.max_neighbors = self.max_neighbors
         c.pair_unroll = self.pair_unroll
         c.lds_budget = self.lds_budget
+        c.split_rebuild = int(self.split_rebuild)
         return c
 
 

@@ -245,3 +245,22 @@ def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged():
     d = p1 - p0
     d -= np.round(d / L) * L
     assert np.abs(d).max() < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fused_rebuild_equals_separate_kernels(prec):
+    # the one-launch rebuild and the nine-kernel rebuild build the same sort
+    # and the same rows: trajectories are bitwise identical
+    s = fcc(6)
+    say("fused rebuild", prec)
+    out = []
+    for split in (False, True):
+        cfg = FastPipelineConfig(precision=prec, rebuild_every=5, split_rebuild=split)
+        sched = FastPipelineScheduler.for_system(s, potential("johnson"), cfg)
+        sched.run_until(30)
+        pos, vel, f, _, _ = sched.download()
+        out.append((pos, vel, f, sched.stats()))
+        sched.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][3].rebuilds == out[1][3].rebuilds == 1 + 5
+    assert out[0][3].kernel_launches < out[1][3].kernel_launches
