@@ -160,9 +160,13 @@ struct BoxGeo {
   int pad;
 };
 
-// ELL index of the k-th neighbour of atom i.
+// ELL index of the k-th neighbour of atom i. A wavefront's 64 rows are
+// stored together, in quads: slots 4q..4q+3 of one row are adjacent (16
+// bytes), and quad q of the 64 rows follows lane order, so a wave reads
+// four slots of each of its rows with one coalesced 1 KiB load. cap is a
+// multiple of 4.
 __device__ __forceinline__ size_t ell(int i, int k, int cap) {
-  return (static_cast<size_t>(i >> 6) * cap + k) * 64 + (i & 63);
+  return ((static_cast<size_t>(i >> 6) * (cap >> 2) + (k >> 2)) * 64 + (i & 63)) * 4 + (k & 3);
 }
 
 // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous

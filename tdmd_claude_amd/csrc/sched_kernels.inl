@@ -250,13 +250,31 @@ __global__ void k_nhc_commit(double* nhc) {
 constexpr int kRowPad = 8;  // rows padded to a multiple of this (cap too)
 constexpr int kPairUnroll = 8;
 
+// Slots k..k+kU-1 of atom i (k a multiple of 4): one 16-byte load per quad.
+template <int kU>
+__device__ __forceinline__ void load_slots(const StepArgs& a, int i, int k, int (&jn)[kU]) {
+  const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
+#pragma unroll
+  for (int q = 0; q < kU; q += 4) {
+    const int4 v = q4[ell(i, k + q, a.cap) >> 2];
+    jn[q] = v.x;
+    jn[q + 1] = v.y;
+    jn[q + 2] = v.z;
+    jn[q + 3] = v.w;
+  }
+}
+
 template <class T, int kU, class Pair>
 __device__ __forceinline__ void for_neighbours(const StepArgs& a, int i, int n,
                                                const T* __restrict__ rec, Pair&& pair) {
-  static_assert(kRowPad % kU == 0, "groups must tile the padded row");
+  static_assert(kRowPad % kU == 0 && kU % 4 == 0, "groups must tile the padded row in quads");
   int jn[kU];
+  if (n > 0) {
+    load_slots<kU>(a, i, 0, jn);
+  } else {
 #pragma unroll
-  for (int u = 0; u < kU; ++u) jn[u] = n > 0 ? a.nbr[ell(i, u, a.cap)] : i;
+    for (int u = 0; u < kU; ++u) jn[u] = i;
+  }
   for (int k = 0; k < n; k += kU) {
     int j[kU];
     T mj[kU];
@@ -265,10 +283,7 @@ __device__ __forceinline__ void for_neighbours(const StepArgs& a, int i, int n,
       j[u] = jn[u];
       mj[u] = rec[j[u]];
     }
-    if (k + kU < n) {
-#pragma unroll
-      for (int u = 0; u < kU; ++u) jn[u] = a.nbr[ell(i, k + kU + u, a.cap)];
-    }
+    if (k + kU < n) load_slots<kU>(a, i, k + kU, jn);
 #pragma unroll
     for (int u = 0; u < kU; ++u) pair(j[u], mj[u], k + u < n);
   }

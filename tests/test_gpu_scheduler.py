@@ -264,3 +264,45 @@ def test_fused_rebuild_equals_separate_kernels(prec):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert out[0][3].rebuilds == out[1][3].rebuilds == 1 + 5
     assert out[0][3].kernel_launches < out[1][3].kernel_launches
+
+
+def test_long_cutoff_morse_matches_oracle():
+    # a 13 A Morse cutoff: ~290 atoms per cell, ~1000 neighbours per row and
+    # three cells per axis, so the 27 cells around any atom are the whole
+    # box: rows, padding and capacity sizing far from the metal benchmarks
+    s = fcc(13)
+    morse = MORSE_CU[:3] + (13.0,)
+    say("long-cutoff morse", s.natoms)
+    sched = FastPipelineScheduler.for_system(s, morse, FastPipelineConfig(precision="fp64"))
+    sched.run_until(12)
+    pos, vel, f, _, _ = sched.download()
+    rpos, rvel, rf, _, reb = O.run(s, morse=morse, nsteps=12, thermo_every=12, rebuild_mode=1,
+                                   rebuild_every=10)
+    L = s.hi - s.lo
+    dpos = pos - rpos
+    dpos -= np.round(dpos / L) * L
+    say("  max dpos", np.abs(dpos).max(), "force err", force_rel_err(f, rf))
+    assert np.abs(dpos).max() < 1e-10
+    assert force_rel_err(f, rf) < 1e-9
+    assert sched.stats().rebuilds == 2  # initial + step 11
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_open_axis_slab_matches_oracle(prec):
+    # (001) slab: z open with vacuum above and below the crystal
+    s = fcc(6)
+    s.lo[2] -= 6.0
+    s.hi[2] += 6.0
+    s.periodic = (1, 1, 0)
+    say("open-z slab", prec)
+    sched, pos, vel, f = gpu_run(s, "johnson", 40, precision=prec)
+    rpos, rvel, rf, _, _ = oracle_run(s, "johnson", 40)
+    L = s.hi - s.lo
+    dpos = pos - rpos
+    dpos[:, :2] -= np.round(dpos[:, :2] / L[:2]) * L[:2]
+    err = force_rel_err(f, rf)
+    say("  max dpos", np.abs(dpos).max(), "force err", err)
+    if prec == "fp64":
+        assert np.abs(dpos).max() < 1e-10 and err < 1e-9
+    else:
+        assert np.abs(dpos).max() < 1e-5 and err < 1e-4
