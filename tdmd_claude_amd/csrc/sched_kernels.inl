@@ -623,33 +623,44 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
   const CellGrid cg = grid_of<Mir>(a, g, i);
   int c[3];
   cell_coords(cg, xi, yi, zi, c);
-  for (int dz = -1; dz <= 1; ++dz) {
-    int nz = c[2] + dz;
+  // The 27 cells around c, visited in increasing rank. Atoms are sorted by
+  // cell rank, so each row comes out in increasing atom index, and the
+  // lanes of a wave (neighbouring atoms with overlapping neighbourhoods)
+  // reach the same stretch of the record array at the same slot.
+  int rk[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) {
+    int nx = c[0] + t % 3 - 1, ny = c[1] + (t / 3) % 3 - 1, nz = c[2] + t / 9 - 1;
+    nx = nx < 0 ? nx + cg.n[0] : (nx >= cg.n[0] ? nx - cg.n[0] : nx);
+    ny = ny < 0 ? ny + cg.n[1] : (ny >= cg.n[1] ? ny - cg.n[1] : ny);
     nz = nz < 0 ? nz + cg.n[2] : (nz >= cg.n[2] ? nz - cg.n[2] : nz);
-    for (int dy = -1; dy <= 1; ++dy) {
-      int ny = c[1] + dy;
-      ny = ny < 0 ? ny + cg.n[1] : (ny >= cg.n[1] ? ny - cg.n[1] : ny);
-      for (int dx = -1; dx <= 1; ++dx) {
-        int nx = c[0] + dx;
-        nx = nx < 0 ? nx + cg.n[0] : (nx >= cg.n[0] ? nx - cg.n[0] : nx);
-        const int r = g.rank_of[cg.off + (nz * cg.n[1] + ny) * cg.n[0] + nx];
-        const int start = g.cstart[r], end = start + g.ccount[r];
-        for (int j = start; j < end; ++j) {
-          if (j == i) continue;
-          const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
-          const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
-          const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
-          const double r2 = r2_unfused(ddx, ddy, ddz);
-          if (r2 < r_list_sq) {
-            if (!kCount && nn + nf < cap) {
-              if (r2 < near_sq)
-                nbr[ell(i, nn++, cap)] = j;
-              else
-                nbr[ell(i, cap - 1 - nf++, cap)] = j;
-            }
-            ++count;
-          }
+    rk[t] = g.rank_of[cg.off + (nz * cg.n[1] + ny) * cg.n[0] + nx];
+  }
+  unsigned left = (1u << 27) - 1u;
+  for (int step = 0; step < 27; ++step) {
+    int r = 0x7fffffff, bt = 0;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const bool take = ((left >> t) & 1u) != 0u && rk[t] < r;
+      r = take ? rk[t] : r;
+      bt = take ? t : bt;
+    }
+    left &= ~(1u << bt);
+    const int start = g.cstart[r], end = start + g.ccount[r];
+    for (int j = start; j < end; ++j) {
+      if (j == i) continue;
+      const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
+      const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
+      const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
+      const double r2 = r2_unfused(ddx, ddy, ddz);
+      if (r2 < r_list_sq) {
+        if (!kCount && nn + nf < cap) {
+          if (r2 < near_sq)
+            nbr[ell(i, nn++, cap)] = j;
+          else
+            nbr[ell(i, cap - 1 - nf++, cap)] = j;
         }
+        ++count;
       }
     }
   }
