@@ -40,6 +40,7 @@ struct FastPipelineConfig {
   i32 pair_unroll{0};
   i32 lds_budget{0};
   bool split_rebuild{false};
+  bool disable_halo{false};  // no LDS halo staging in the EAM passes
 };
 
 struct FastPipelineStats {
@@ -155,6 +156,7 @@ class FastPipelineScheduler {
     c.pair_unroll = cfg.pair_unroll;
     c.lds_budget = cfg.lds_budget;
     c.split_rebuild = cfg.split_rebuild ? 1 : 0;
+    c.disable_halo = cfg.disable_halo ? 1 : 0;
     return c;
   }
 

@@ -273,7 +273,9 @@ typedef struct {
                           /* 0 = device limit                             */
   int32_t split_rebuild;  /* 1: neighbour rebuild as separate kernels;    */
                           /* 0 (default): one fused launch per step       */
-  int32_t reserved[5];
+  int32_t disable_halo;   /* 1: no LDS halo staging in the single-element */
+                          /* EAM passes (0: staged where records are 16 B)*/
+  int32_t reserved[4];
 } tdmd_sched_config;
 
 void tdmd_sched_config_default(tdmd_sched_config* cfg);
@@ -346,6 +348,11 @@ int tdmd_sched_time_passes(tdmd_sched* s, int32_t steps, float* ms_out);
  * pair_unroll, out[1] = lds_budget bytes (0 where the potential has no LDS
  * window to choose). Runs the initial force pass if needed. */
 int tdmd_sched_autotune(tdmd_sched* s, int32_t iters, int32_t out[2]);
+/* Halo staging (no reference counterpart; diagnostics): out[0] = blocks of
+ * 1024 atoms, out[1] = blocks whose halo is staged in LDS, out[2] = largest
+ * halo (atoms), out[3] = halo atoms summed over all blocks. All zero when
+ * the passes do not stage halos. Synchronises the scheduler's stream. */
+int tdmd_sched_halo_info(tdmd_sched* s, int64_t out[4]);
 /* Stream the scheduler runs on (for event timing). */
 int tdmd_sched_stream(const tdmd_sched* s, void** stream);
 

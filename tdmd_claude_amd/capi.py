@@ -84,7 +84,8 @@ class SchedConfig(C.Structure):
         ("pair_unroll", C.c_int32),
         ("lds_budget", C.c_int32),
         ("split_rebuild", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("disable_halo", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -170,6 +171,7 @@ _PROTOS = {
     "tdmd_sched_interaction_cutoff": (C.c_int, [_VP, _PD]),
     "tdmd_sched_thermo": (C.c_int, [_VP, _PD]),
     "tdmd_sched_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "tdmd_sched_halo_info": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
     "tdmd_sched_time_passes": (C.c_int, [_VP, _I32, C.POINTER(C.c_float)]),
     "tdmd_sched_autotune": (C.c_int, [_VP, _I32, _PI32]),
     "tdmd_fm_create": (C.c_int, [C.POINTER(_VP), _I32, C.POINTER(Box), _I32, _I32, _PD, _D]),

@@ -169,6 +169,12 @@ __device__ __forceinline__ size_t ell(int i, int k, int cap) {
   return ((static_cast<size_t>(i >> 6) * (cap >> 2) + (k >> 2)) * 64 + (i & 63)) * 4 + (k & 3);
 }
 
+// Slot k of atom i in the 16-bit rows: eight slots of a row per 16 bytes,
+// the 64 rows of a wavefront side by side (cap is a multiple of 8).
+__device__ __forceinline__ size_t ell16(int i, int k, int cap) {
+  return ((static_cast<size_t>(i >> 6) * (cap >> 3) + (k >> 3)) * 64 + (i & 63)) * 8 + (k & 7);
+}
+
 // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
 // run of atoms (spatially sorted) so neighbour gathers stay in its L2.
 __device__ __forceinline__ int xcd_block(int nblocks) {
@@ -200,9 +206,18 @@ struct StepArgs {
   const double* masses;  // by 1-based type
   const int32_t* nbr;
   const int32_t* cnt;
+  // Halos of the LDS-staged passes (blocks of 1024 atoms): halo b lists the
+  // distinct atoms the rows of block b name, in increasing index; nbr16
+  // holds those rows as positions in the list.
+  const uint16_t* nbr16;
+  const int32_t* hj;     // [blocks][kHaloMax]
+  const int32_t* hn;     // [blocks] halo size
+  const int32_t* hmode;  // [blocks] 0: staged in LDS, 1: reads the 32-bit rows
+  int stage_cap;         // halo records a pass holds in LDS (0: no staging)
   void* M;
   void* M2;
-  int32_t* flags;        // [0] rebuild, [1] overflow max, [2] nonfinite
+  int32_t* flags;        // [0] rebuild, [1] overflow max, [2] nonfinite,
+                         // [3] barrier timeout, [4] halos stale
   double* partials;      // per-block FP64 partial sums
   double* nhc;           // device NHC state block
   // batches of boxes (ManyBoxes mirrors only)

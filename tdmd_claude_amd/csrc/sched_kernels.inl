@@ -289,6 +289,49 @@ __device__ __forceinline__ void for_neighbours(const StepArgs& a, int i, int n,
   }
 }
 
+// Slots k..k+kU-1 of atom i in the 16-bit rows (k a multiple of kU).
+template <int kU>
+__device__ __forceinline__ void load_local(const StepArgs& a, int i, int k, int (&ln)[kU]) {
+  static_assert(kU == 4 || kU == 8, "four or eight slots per group");
+  if constexpr (kU == 8) {
+    const uint4 v = reinterpret_cast<const uint4*>(a.nbr16)[ell16(i, k, a.cap) >> 3];
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ln[2 * q] = static_cast<int>(w[q] & 0xffffu);
+      ln[2 * q + 1] = static_cast<int>(w[q] >> 16);
+    }
+  } else {
+    const uint2 v = reinterpret_cast<const uint2*>(a.nbr16)[ell16(i, k, a.cap) >> 2];
+    ln[0] = static_cast<int>(v.x & 0xffffu);
+    ln[1] = static_cast<int>(v.x >> 16);
+    ln[2] = static_cast<int>(v.y & 0xffffu);
+    ln[3] = static_cast<int>(v.y >> 16);
+  }
+}
+
+// for_neighbours over a block whose halo records sit in LDS (`hrec`): the
+// 16-bit rows index the halo, and the records come from LDS.
+template <class T, int kU, class Pair>
+__device__ __forceinline__ void for_neighbours_staged(const StepArgs& a, int i, int n,
+                                                      const T* hrec, Pair&& pair) {
+  int ln[kU];
+  if (n > 0) {
+    load_local<kU>(a, i, 0, ln);
+  } else {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) ln[u] = 0;
+  }
+  for (int k = 0; k < n; k += kU) {
+    T mj[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) mj[u] = hrec[ln[u]];
+    if (k + kU < n) load_local<kU>(a, i, k + kU, ln);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) pair(ln[u], mj[u], k + u < n);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // EAM passes (global tables; any number of elements)
 // ---------------------------------------------------------------------------
@@ -761,6 +804,7 @@ __global__ void __launch_bounds__(kBlock) k_rebuild_done(int32_t* flags, int32_t
     const int prev = atomicAdd(done, 1);
     if (prev == nblocks - 1) {
       *done = 0;
+      flags[4] = 1;  // the rows changed: halos are rebuilt next
       flags[0] = 0;
       *rebuilds += 1;
     }
@@ -851,8 +895,141 @@ k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double nea
   report_count<false>(a, count, nullptr);
   // every thread read the flag at entry, before the first barrier
   if (tid == 0) {
+    a.flags[4] = 1;  // the rows changed: halos are rebuilt next
     a.flags[0] = 0;
     *rebuilds += 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Halos of the LDS-staged passes. Block b of those passes covers atoms
+// [1024 b, 1024 b + 1024); its halo is the set of atoms its rows name (the
+// block's own atoms included), in increasing index. A pass copies the halo's
+// records into LDS once and gathers neighbour records from there, so a
+// block reads ~3.5 records per atom from L2 instead of one per row slot.
+//
+// Built after every rebuild (flags[4]) by one workgroup per block at a
+// time: an LDS hash set collects the distinct indices of the block's rows,
+// a bitonic sort orders them, the list is written out, and every row slot
+// is rewritten as its position in the list (binary search). A halo larger
+// than stage_cap (or than kHaloMax) leaves the block on its 32-bit rows.
+// The last workgroup to finish clears flags[4].
+// ---------------------------------------------------------------------------
+constexpr int kHaloHashBits = 13;
+constexpr int kHaloHash = 1 << kHaloHashBits;  // hash slots
+constexpr int kHaloMax = 6144;                 // distinct atoms a halo may hold
+static_assert(kHaloMax <= 65536 && 4 * kHaloMax <= 3 * kHaloHash, "halo sizing");
+
+// Insert j into the open-addressing set `key` (empty = -1); multiplicative
+// (Fibonacci) hashing, linear probing. Counts new keys in *nd and raises
+// *ovf once more than kHaloMax are held; inserts stop then, so the table
+// never holds more than kHaloMax + (threads of the workgroup) keys and a
+// probe always meets its key or an empty slot.
+__device__ __forceinline__ void halo_insert(int32_t* key, int j, int* nd, volatile int* ovf) {
+  unsigned h = (static_cast<unsigned>(j) * 2654435761u) >> (32 - kHaloHashBits);
+  for (int p = 0; p < kHaloHash; ++p) {
+    const int old = atomicCAS(&key[h], -1, j);
+    if (old == j) return;
+    if (old == -1) {
+      if (atomicAdd(nd, 1) >= kHaloMax) *ovf = 1;
+      return;
+    }
+    h = (h + 1) & (kHaloHash - 1);
+  }
+  *ovf = 1;
+}
+
+constexpr int kHaloBlock = 1024;  // atoms per staged block (= the passes' workgroup)
+
+__global__ void __launch_bounds__(kHaloBlock) k_halo(StepArgs a, int nblocks, int32_t* done) {
+  if (!a.flags[4]) return;
+  __shared__ int32_t key[kHaloHash];
+  __shared__ int32_t srt[kHaloHash];
+  __shared__ int nd, fill;
+  __shared__ int ovf;
+  const int tid = threadIdx.x;
+  const int cap = a.cap;
+  int32_t* hj = const_cast<int32_t*>(a.hj);
+  int32_t* hn = const_cast<int32_t*>(a.hn);
+  int32_t* hmode = const_cast<int32_t*>(a.hmode);
+  uint16_t* nbr16 = const_cast<uint16_t*>(a.nbr16);
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    for (int t = tid; t < kHaloHash; t += kHaloBlock) key[t] = -1;
+    if (tid == 0) {
+      nd = 0;
+      fill = 0;
+      ovf = 0;
+    }
+    __syncthreads();
+    const int i = b * kHaloBlock + tid;
+    const bool valid = i < a.natoms;
+    const int n = valid ? a.cnt[i] : 0;
+    volatile int* vovf = &ovf;
+    if (valid) halo_insert(key, i, &nd, vovf);  // row padding names i itself
+    for (int k = 0; k < n && !*vovf; ++k) halo_insert(key, a.nbr[ell(i, k, cap)], &nd, vovf);
+    __syncthreads();
+    const int H = nd;
+    if (ovf || H > a.stage_cap) {  // the same in every thread
+      if (tid == 0) {
+        hn[b] = H;
+        hmode[b] = 1;
+      }
+      __syncthreads();  // LDS is reused by the next block
+      continue;
+    }
+    for (int t = tid; t < kHaloHash; t += kHaloBlock) {
+      const int v = key[t];
+      if (v != -1) srt[atomicAdd(&fill, 1)] = v;
+    }
+    int P = 1;
+    while (P < H) P <<= 1;
+    __syncthreads();
+    for (int t = H + tid; t < P; t += kHaloBlock) srt[t] = 0x7fffffff;
+    __syncthreads();
+    // bitonic sort of srt[0, P), ascending
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int t = tid; t < (P >> 1); t += kHaloBlock) {
+          const int lo = 2 * t - (t & (stride - 1));
+          const int hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const int x = srt[lo], y = srt[hi];
+          if ((x > y) == up) {
+            srt[lo] = y;
+            srt[hi] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int l = tid; l < H; l += kHaloBlock) hj[static_cast<size_t>(b) * kHaloMax + l] = srt[l];
+    if (valid) {
+      const int m = (n + 7) & ~7;  // the padded row
+      for (int k = 0; k < m; ++k) {
+        const int j = a.nbr[ell(i, k, cap)];
+        int lo = 0, hi = H - 1;  // j is in srt[lo, hi]
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (srt[mid] < j)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        nbr16[ell16(i, k, cap)] = static_cast<uint16_t>(lo);
+      }
+    }
+    if (tid == 0) {
+      hn[b] = H;
+      hmode[b] = 0;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(done, 1) == static_cast<int>(gridDim.x) - 1) {
+      *done = 0;
+      a.flags[4] = 0;
+    }
   }
 }
 

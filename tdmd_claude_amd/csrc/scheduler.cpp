@@ -163,6 +163,13 @@ struct Engine final : EngineBase {
   // limit unless the configuration asks for less, e.g. to fit two
   // workgroups per CU).
   bool lds = false;
+  // Halo staging (k_halo + the kStage passes) for 16-byte gather records.
+  static constexpr bool kCanStage = sizeof(typename Mir::T) == 16;
+  static constexpr int kStageCap = 4608;  // halo records per block in LDS (72 KiB)
+  bool stage = false;
+  int ncu = 0;
+  int32_t* d_halo_done = nullptr;
+  size_t stage_bytes() const { return stage ? static_cast<size_t>(kStageCap) * 16 : 0; }
   int unroll = 4;
   int lds_limit = 0;  // per-workgroup dynamic LDS the kernels are allowed
   int nint = 0;       // r-grid intervals
@@ -174,7 +181,7 @@ struct Engine final : EngineBase {
     LdsWindow w;
     w.nwin = static_cast<int>(std::min<size_t>(static_cast<size_t>(nint), budget / per));
     w.k_lo = nint - w.nwin;
-    *bytes = static_cast<size_t>(w.nwin) * per;
+    *bytes = window_bytes(w.nwin, per);
     return w;
   }
   template <class K>
@@ -185,29 +192,38 @@ struct Engine final : EngineBase {
     return TDMD_OK;
   }
 
-  template <bool kWin, int kU>
+  template <bool kWin, int kU, bool kSt>
   int allow_all() {
     int rc = 0;
     const size_t L = static_cast<size_t>(lds_limit);
-    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin, kU>, L)) ||
-        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin, kU>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin, kU>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin, kU>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU>, L)))
+    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin, kU, kSt>, L)) ||
+        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin, kU, kSt>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin, kU, kSt>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin, kU, kSt>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU, kSt>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU, kSt>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU, kSt>, L)))
+      return rc;
+    return TDMD_OK;
+  }
+  template <bool kSt>
+  int allow_shapes() {
+    int rc = 0;
+    if ((rc = allow_all<false, 4, kSt>()) || (rc = allow_all<true, 4, kSt>()) ||
+        (rc = allow_all<false, 8, kSt>()) || (rc = allow_all<true, 8, kSt>()))
       return rc;
     return TDMD_OK;
   }
 
   // Windows for a budget (bytes, clamped to the allowed limit) and unroll.
+  // With halo staging the windows get what the halo leaves.
   void configure(int u, int bytes) {
     budget = std::min(bytes > 0 ? bytes : lds_limit, lds_limit);
+    const size_t wb = static_cast<size_t>(budget) - std::min(static_cast<size_t>(budget), stage_bytes());
     using V2 = typename Vec2<R>::type;
     for (int e = 0; e < 2; ++e) {
-      w_dens[e] = window(nint, sizeof(Q), static_cast<size_t>(budget), &b_dens[e]);
-      w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0),
-                          static_cast<size_t>(budget), &b_force[e]);
+      w_dens[e] = window(nint, sizeof(Q), wb, &b_dens[e]);
+      w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0), wb, &b_force[e]);
     }
     unroll = u;
   }
@@ -242,14 +258,44 @@ struct Engine final : EngineBase {
     if (S->cfg.lds_budget > 0 && S->cfg.lds_budget < 4096)
       return fail(TDMD_E_INVALID, "scheduler: lds_budget below 4096 bytes");
     nint = m[1].n - 1;
-    configure(S->cfg.pair_unroll == 0 ? 8 : S->cfg.pair_unroll, S->cfg.lds_budget);
     int rc = 0;
-    if ((rc = allow_all<false, 4>()) || (rc = allow_all<true, 4>()) ||
-        (rc = allow_all<false, 8>()) || (rc = allow_all<true, 8>()))
-      return rc;
+    if ((rc = allow_shapes<false>())) return rc;
     lds = true;
     S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
+    if constexpr (kCanStage) {
+      if (!S->cfg.disable_halo && lds_limit >= static_cast<int>(kStageCap) * 16 + 16384) {
+        if ((rc = allow_shapes<true>())) return rc;
+        TDMD_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const size_t nbf = static_cast<size_t>(S->nb_force);
+        void* p = nullptr;
+        const size_t sizes[4] = {nbf * kHaloMax * 4, nbf * 4, nbf * 4, 4};
+        void* ptrs[4];
+        for (int k = 0; k < 4; ++k) {
+          if ((rc = tdmd_malloc(&p, sizes[k]))) return rc;
+          S->allocs.push_back(p);
+          ptrs[k] = p;
+        }
+        S->a.hj = static_cast<const int32_t*>(ptrs[0]);
+        S->a.hn = static_cast<const int32_t*>(ptrs[1]);
+        S->a.hmode = static_cast<const int32_t*>(ptrs[2]);
+        d_halo_done = static_cast<int32_t*>(ptrs[3]);
+        TDMD_HIP_TRY(hipMemset(ptrs[2], 0xff, sizes[2]));  // no halo yet: not staged
+        TDMD_HIP_TRY(hipMemset(d_halo_done, 0, 4));
+        S->a.stage_cap = kStageCap;
+        stage = true;
+      }
+    }
+    configure(S->cfg.pair_unroll == 0 ? 8 : S->cfg.pair_unroll, S->cfg.lds_budget);
     return TDMD_OK;
+  }
+
+  // Halos of the staged passes, after a rebuild (returns at once otherwise).
+  void halo(hipStream_t s) {
+    if (!stage) return;
+    const int grid = std::max(1, std::min(S->nb_force, 2 * std::max(ncu, 1)));
+    k_halo<<<grid, kHaloBlock, 0, s>>>(S->a, (S->natoms + kHaloBlock - 1) / kHaloBlock,
+                                       d_halo_done);
+    ++S->launches;
   }
 
   int tune(int iters, int32_t* out) override {
@@ -265,9 +311,12 @@ struct Engine final : EngineBase {
     };
     const Cand cands[] = {{8, lds_limit}, {4, lds_limit}, {8, 80 * 1024 - 1024},
                           {4, 80 * 1024 - 1024}};
+    // with halo staging the halo already takes half the LDS: unroll only
+    const int ncand = stage ? 2 : 4;
     float best = 1e30f;
     Cand pick = cands[0];
-    for (const Cand& c : cands) {
+    for (int ci = 0; ci < ncand; ++ci) {
+      const Cand& c = cands[ci];
       configure(c.u, c.b);
       // density + force with the force-only epilogue: both passes rewrite
       // M2 and F with the values they already hold, so timing is free of
@@ -302,8 +351,16 @@ struct Engine final : EngineBase {
   template <bool kEnergy, bool kWin, int kU>
   void density_lds(hipStream_t s) {
     const int nb = S->nb_force;
-    go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU>, nb, kBigBlock, b_dens[kEnergy], s, S->a,
-       meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
+    if constexpr (kCanStage) {
+      if (stage) {
+        go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, true>, nb, kBigBlock,
+           b_dens[kEnergy] + stage_bytes(), s, S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy],
+           nb);
+        return;
+      }
+    }
+    go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, false>, nb, kBigBlock, b_dens[kEnergy], s,
+       S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
   }
   template <bool kEnergy>
   void density(hipStream_t s) {
@@ -322,8 +379,16 @@ struct Engine final : EngineBase {
   template <int kEpi, bool kEnergy, bool kWin, int kU>
   void force_lds(hipStream_t s) {
     const int nb = S->nb_force;
-    go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU>, nb, kBigBlock, b_force[kEnergy], s, S->a,
-       meta(), coeff(), rc_sq(), w_force[kEnergy], nb);
+    if constexpr (kCanStage) {
+      if (stage) {
+        go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, true>, nb, kBigBlock,
+           b_force[kEnergy] + stage_bytes(), s, S->a, meta(), coeff(), rc_sq(),
+           w_force[kEnergy], nb);
+        return;
+      }
+    }
+    go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, false>, nb, kBigBlock, b_force[kEnergy],
+       s, S->a, meta(), coeff(), rc_sq(), w_force[kEnergy], nb);
   }
 
   template <int kEpi, bool kEnergy>
@@ -355,6 +420,7 @@ struct Engine final : EngineBase {
           const_cast<int32_t*>(S->a.nbr), const_cast<int32_t*>(S->a.cnt), S->d_bar,
           S->d_rebuilds);
       ++S->launches;
+      halo(s);
       TDMD_LAUNCH_CHECK();
       return TDMD_OK;
     }
@@ -383,6 +449,7 @@ struct Engine final : EngineBase {
                                              const_cast<int32_t*>(S->a.cnt), nullptr);
     k_rebuild_done<<<gb, kBlock, 0, s>>>(S->a.flags, S->g.done, gb, S->d_rebuilds);
     S->launches += 2;
+    halo(s);
     TDMD_LAUNCH_CHECK();
     return TDMD_OK;
   }
@@ -711,6 +778,7 @@ void sched_free(tdmd_sched* S) {
   for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
   for (void* p : S->allocs) (void)hipFree(p);
   if (S->a.nbr) (void)hipFree(const_cast<int32_t*>(S->a.nbr));
+  if (S->a.nbr16) (void)hipFree(const_cast<uint16_t*>(S->a.nbr16));
   if (S->tables) eam_tables_destroy(S->tables);
   if (S->stream) (void)hipStreamDestroy(S->stream);
 }
@@ -748,10 +816,17 @@ int first_build(tdmd_sched* S) {
   cap = round_up(std::max(cap, kRowPad), kRowPad);  // rows are padded to kRowPad slots
   if (S->a.nbr == nullptr || cap > S->a.cap) {
     if (S->a.nbr) TDMD_HIP_TRY(hipFree(const_cast<int32_t*>(S->a.nbr)));
+    if (S->a.nbr16) TDMD_HIP_TRY(hipFree(const_cast<uint16_t*>(S->a.nbr16)));
+    S->a.nbr = nullptr;
+    S->a.nbr16 = nullptr;
+    const size_t slots = static_cast<size_t>(round_up(S->natoms, 64)) * cap;
     void* p = nullptr;
-    rc = tdmd_malloc(&p, static_cast<size_t>(round_up(S->natoms, 64)) * cap * 4);
-    if (rc) return rc;
+    if ((rc = tdmd_malloc(&p, slots * 4))) return rc;
     S->a.nbr = static_cast<int32_t*>(p);
+    if (S->a.stage_cap > 0) {  // 16-bit rows of the staged passes
+      if ((rc = tdmd_malloc(&p, slots * 2))) return rc;
+      S->a.nbr16 = static_cast<uint16_t*>(p);
+    }
     S->a.cap = cap;
     for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
@@ -1223,6 +1298,24 @@ int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
   S->steps_since += steps;
   S->ticks += steps;
   return check_flags(S);
+}
+
+int tdmd_sched_halo_info(tdmd_sched* S, int64_t out[4]) {
+  if (!S || !out) return fail(TDMD_E_INVALID, "sched_halo_info: null");
+  for (int k = 0; k < 4; ++k) out[k] = 0;
+  if (S->a.stage_cap == 0) return TDMD_OK;
+  const int nb = (S->natoms + 1023) / 1024;
+  std::vector<int32_t> hn(static_cast<size_t>(nb)), mode(static_cast<size_t>(nb));
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  TDMD_HIP_TRY(hipMemcpy(hn.data(), S->a.hn, hn.size() * 4, hipMemcpyDeviceToHost));
+  TDMD_HIP_TRY(hipMemcpy(mode.data(), S->a.hmode, mode.size() * 4, hipMemcpyDeviceToHost));
+  out[0] = nb;
+  for (int b = 0; b < nb; ++b) {
+    out[1] += mode[static_cast<size_t>(b)] == 0;
+    out[2] = std::max<int64_t>(out[2], hn[static_cast<size_t>(b)]);
+    out[3] += hn[static_cast<size_t>(b)];
+  }
+  return TDMD_OK;
 }
 
 int tdmd_sched_stream(const tdmd_sched* S, void** stream) {

@@ -42,6 +42,7 @@ class FastPipelineConfig:
     pair_unroll: int = 0  # neighbour slots per pipelined group (0 = default)
     lds_budget: int = 0  # LDS bytes per workgroup for spline windows (0 = device limit)
     split_rebuild: bool = False  # rebuild as separate kernels instead of one fused launch
+    disable_halo: bool = False  # no LDS halo staging in the single-element EAM passes
 
     def to_c(self) -> capi.SchedConfig:
         c = capi.SchedConfig()
@@ -61,6 +62,7 @@ class FastPipelineConfig:
         c.pair_unroll = self.pair_unroll
         c.lds_budget = self.lds_budget
         c.split_rebuild = int(self.split_rebuild)
+        c.disable_halo = int(self.disable_halo)
         return c
 
 
@@ -178,6 +180,13 @@ class FastPipelineScheduler:
         out = (C.c_int32 * 2)()
         capi.call("tdmd_sched_autotune", self._h, int(iters), out)
         return {"pair_unroll": int(out[0]), "lds_budget": int(out[1])}
+
+    def halo_info(self) -> dict:
+        """LDS halo staging: 1024-atom blocks, blocks staged, largest and total halo."""
+        out = (C.c_int64 * 4)()
+        capi.call("tdmd_sched_halo_info", self._h, out)
+        return {"blocks": int(out[0]), "staged": int(out[1]), "max_halo": int(out[2]),
+                "halo_atoms": int(out[3])}
 
     def stream(self) -> C.c_void_p:
         s = C.c_void_p()

@@ -306,3 +306,32 @@ def test_open_axis_slab_matches_oracle(prec):
         assert np.abs(dpos).max() < 1e-10 and err < 1e-9
     else:
         assert np.abs(dpos).max() < 1e-5 and err < 1e-4
+
+
+@pytest.mark.parametrize("r_c", [None, 8.5])
+def test_halo_staging_is_bitwise_neutral(r_c):
+    # staging changes where the passes read neighbour records from (LDS), not
+    # which records or in what order: trajectories with and without it are
+    # bitwise identical. At r_c 8.5 A a block's halo (~6000 atoms) exceeds
+    # the LDS capacity and the blocks fall back to their 32-bit rows.
+    s = fcc(12)  # 6912 atoms: six full blocks of 1024 and one of 768
+    tables = (potential("johnson") if r_c is None else
+              I.johnson_setfl(I.JohnsonParams(r_c=r_c), nr=2000, nrho=2000))
+    say("halo staging", r_c)
+    out = []
+    for off in (False, True):
+        cfg = FastPipelineConfig(precision="mixed", rebuild_every=5, disable_halo=off)
+        sched = FastPipelineScheduler.for_system(s, tables, cfg)
+        sched.run_until(20)
+        pos, vel, f, _, _ = sched.download()
+        out.append((pos, vel, f, sched.halo_info()))
+        sched.close()
+    info = out[0][3]
+    say("  halos", info)
+    assert info["blocks"] == 7 and out[1][3]["blocks"] == 0
+    if r_c is None:
+        assert info["staged"] == 7 and info["max_halo"] <= 4608
+    else:
+        assert info["staged"] < 7
+    for k in range(3):
+        assert np.array_equal(out[0][k], out[1][k])
