@@ -252,81 +252,105 @@ constexpr int kPairUnroll = 8;
 
 // Slots k..k+kU-1 of atom i (k a multiple of 4): one 16-byte load per quad.
 template <int kU>
-__device__ __forceinline__ void load_slots(const StepArgs& a, int i, int k, int (&jn)[kU]) {
+__device__ __forceinline__ void load_slots(const StepArgs& a, int i, int k, int4 (&q)[kU / 4]) {
   const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
 #pragma unroll
-  for (int q = 0; q < kU; q += 4) {
-    const int4 v = q4[ell(i, k + q, a.cap) >> 2];
-    jn[q] = v.x;
-    jn[q + 1] = v.y;
-    jn[q + 2] = v.z;
-    jn[q + 3] = v.w;
-  }
+  for (int h = 0; h < kU / 4; ++h) q[h] = q4[ell(i, k + 4 * h, a.cap) >> 2];
 }
 
+// Record j of a gather array, addressed as the array base plus an unsigned
+// 32-bit byte offset (the host keeps natoms x record size below 4 GiB), so
+// the index stays one 32-bit register instead of a sign-extended pair.
+template <class T>
+__device__ __forceinline__ T gather(const T* __restrict__ rec, int j) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(rec) +
+                                     static_cast<uint32_t>(j) * static_cast<uint32_t>(sizeof(T)));
+}
+
+// The loop carries the next group's slots as the loaded quads and unpacks
+// them at the top of the next iteration: registers the loads write are not
+// copied before this group's pair work, so that work overlaps their latency
+// (and the records' gathers) instead of waiting for them.
 template <class T, int kU, class Pair>
 __device__ __forceinline__ void for_neighbours(const StepArgs& a, int i, int n,
                                                const T* __restrict__ rec, Pair&& pair) {
   static_assert(kRowPad % kU == 0 && kU % 4 == 0, "groups must tile the padded row in quads");
-  int jn[kU];
+  int4 q[kU / 4];
   if (n > 0) {
-    load_slots<kU>(a, i, 0, jn);
+    load_slots<kU>(a, i, 0, q);
   } else {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) jn[u] = i;
+    for (int h = 0; h < kU / 4; ++h) q[h] = make_int4(i, i, i, i);
   }
   for (int k = 0; k < n; k += kU) {
     int j[kU];
+#pragma unroll
+    for (int h = 0; h < kU / 4; ++h) {
+      j[4 * h] = q[h].x;
+      j[4 * h + 1] = q[h].y;
+      j[4 * h + 2] = q[h].z;
+      j[4 * h + 3] = q[h].w;
+    }
     T mj[kU];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      j[u] = jn[u];
-      mj[u] = rec[j[u]];
-    }
-    if (k + kU < n) load_slots<kU>(a, i, k + kU, jn);
+    for (int u = 0; u < kU; ++u) mj[u] = gather(rec, j[u]);
+    if (k + kU < n) load_slots<kU>(a, i, k + kU, q);
 #pragma unroll
     for (int u = 0; u < kU; ++u) pair(j[u], mj[u], k + u < n);
   }
 }
 
-// Slots k..k+kU-1 of atom i in the 16-bit rows (k a multiple of kU).
+// Packed 16-bit slots of one group: eight slots in a uint4, four in a uint2.
 template <int kU>
-__device__ __forceinline__ void load_local(const StepArgs& a, int i, int k, int (&ln)[kU]) {
+struct Slots16;
+template <>
+struct Slots16<8> {
+  using type = uint4;
+};
+template <>
+struct Slots16<4> {
+  using type = uint2;
+};
+
+// The packed slots k..k+kU-1 of atom i in the 16-bit rows (k a multiple of kU).
+template <int kU>
+__device__ __forceinline__ typename Slots16<kU>::type load_local(const StepArgs& a, int i, int k) {
   static_assert(kU == 4 || kU == 8, "four or eight slots per group");
-  if constexpr (kU == 8) {
-    const uint4 v = reinterpret_cast<const uint4*>(a.nbr16)[ell16(i, k, a.cap) >> 3];
-    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+  using P = typename Slots16<kU>::type;
+  return reinterpret_cast<const P*>(a.nbr16)[ell16(i, k, a.cap) / kU];
+}
+
+__device__ __forceinline__ void unpack_local(const uint4& v, int (&ln)[8]) {
+  const unsigned w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      ln[2 * q] = static_cast<int>(w[q] & 0xffffu);
-      ln[2 * q + 1] = static_cast<int>(w[q] >> 16);
-    }
-  } else {
-    const uint2 v = reinterpret_cast<const uint2*>(a.nbr16)[ell16(i, k, a.cap) >> 2];
-    ln[0] = static_cast<int>(v.x & 0xffffu);
-    ln[1] = static_cast<int>(v.x >> 16);
-    ln[2] = static_cast<int>(v.y & 0xffffu);
-    ln[3] = static_cast<int>(v.y >> 16);
+  for (int q = 0; q < 4; ++q) {
+    ln[2 * q] = static_cast<int>(w[q] & 0xffffu);
+    ln[2 * q + 1] = static_cast<int>(w[q] >> 16);
   }
+}
+__device__ __forceinline__ void unpack_local(const uint2& v, int (&ln)[4]) {
+  ln[0] = static_cast<int>(v.x & 0xffffu);
+  ln[1] = static_cast<int>(v.x >> 16);
+  ln[2] = static_cast<int>(v.y & 0xffffu);
+  ln[3] = static_cast<int>(v.y >> 16);
 }
 
 // for_neighbours over a block whose halo records sit in LDS (`hrec`): the
-// 16-bit rows index the halo, and the records come from LDS.
+// 16-bit rows index the halo, and the records come from LDS. The next
+// group's slots stay packed until the top of the next iteration, so their
+// load is waited for only after this group's pair work.
 template <class T, int kU, class Pair>
 __device__ __forceinline__ void for_neighbours_staged(const StepArgs& a, int i, int n,
                                                       const T* hrec, Pair&& pair) {
-  int ln[kU];
-  if (n > 0) {
-    load_local<kU>(a, i, 0, ln);
-  } else {
-#pragma unroll
-    for (int u = 0; u < kU; ++u) ln[u] = 0;
-  }
+  typename Slots16<kU>::type raw{};
+  if (n > 0) raw = load_local<kU>(a, i, 0);
   for (int k = 0; k < n; k += kU) {
+    int ln[kU];
+    unpack_local(raw, ln);
     T mj[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) mj[u] = hrec[ln[u]];
-    if (k + kU < n) load_local<kU>(a, i, k + kU, ln);
+    if (k + kU < n) raw = load_local<kU>(a, i, k + kU);
 #pragma unroll
     for (int u = 0; u < kU; ++u) pair(ln[u], mj[u], k + u < n);
   }

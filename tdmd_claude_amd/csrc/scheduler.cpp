@@ -572,6 +572,10 @@ int sched_alloc(tdmd_sched* S) {
   a.masses = masses;
   const bool fixed = S->cfg.precision == TDMD_MIXED && !S->cfg.mirror_fp64;
   const size_t msz = fixed ? 16 : 32;
+  // gathers address records by a 32-bit byte offset
+  if (static_cast<uint64_t>(N) * msz >= (uint64_t{1} << 32))
+    return fail(TDMD_E_INVALID, "scheduler: at most " + std::to_string((uint64_t{1} << 32) / msz - 1) +
+                                    " atoms in one scheduler at this precision");
   char *M, *M2;
   if ((rc = dalloc(S, &M, N * msz))) return rc;
   if ((rc = dalloc(S, &M2, N * msz))) return rc;
