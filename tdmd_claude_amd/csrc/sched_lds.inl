@@ -29,6 +29,7 @@ namespace tdmd_hip {
 namespace sched {
 
 constexpr int kBigBlock = 1024;
+constexpr int kStageCap = 4608;  // halo records a staged pass holds in LDS (72 KiB of 16 B)
 
 struct LdsWindow {
   int k_lo;  // first interval held in LDS
@@ -56,22 +57,94 @@ __device__ __forceinline__ R cubic_deriv(R b, R c, R d, R dx) {
   return b + dx * (static_cast<R>(2) * c + dx * static_cast<R>(3) * d);
 }
 
-__device__ __forceinline__ int big_block_index(int nblocks) {
-  const int b = blockIdx.x;
-  const int per = nblocks >> 3;
-  return (b & 7) * per + (b >> 3);
+// Persistent block loop of the LDS passes. The grid holds a multiple of 8
+// workgroups, at most nblocks (itself a multiple of 8). Workgroup g serves
+// XCD x = g % 8, whose blocks are the contiguous run [x per, (x+1) per),
+// per = nblocks / 8, and takes every (grid / 8)-th block of that run from
+// the (g / 8)-th: neighbouring blocks, and so their halos, stay in one L2.
+// The spline window is copied into LDS once per workgroup.
+//
+// With halo staging the copy of a block's halo is pipelined across the
+// workgroup's blocks: while block b is evaluated, the records of its next
+// block are being gathered into registers and the halo indices of the block
+// after that are loaded, so a block starts with LDS stores only.
+template <class T>
+struct HaloPipe {
+  static constexpr int kPer = (kStageCap + kBigBlock - 1) / kBigBlock;
+  int j[kPer];  // halo indices of the block after next (-1: none)
+  T v[kPer];    // halo records of the next block
+  int jn = 0, jmode = 1;  // halo size and mode of the indices' block
+  int vn = 0, vmode = 1;  // and of the records' block
+
+  __device__ __forceinline__ void load_indices(const StepArgs& a, int blk) {
+    jmode = a.hmode[blk];
+    jn = jmode == 0 ? a.hn[blk] : 0;  // <= stage_cap <= kStageCap when staged
+    const int32_t* __restrict__ hj = a.hj + static_cast<size_t>(blk) * kHaloMax;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int t = static_cast<int>(threadIdx.x) + u * kBigBlock;
+      j[u] = t < jn ? hj[t] : -1;
+    }
+  }
+  __device__ __forceinline__ void none() {
+    jn = 0;
+    jmode = 1;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) j[u] = -1;
+  }
+  __device__ __forceinline__ void gather_records(const T* __restrict__ rec) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+      if (j[u] >= 0) v[u] = gather(rec, j[u]);
+    vn = jn;
+    vmode = jmode;
+  }
+  __device__ __forceinline__ void store(T* hrec) const {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int t = static_cast<int>(threadIdx.x) + u * kBigBlock;
+      if (t < vn) hrec[t] = v[u];
+    }
+  }
+};
+
+// Block sequence of this workgroup (see above).
+struct BlockSeq {
+  int base, first, per, step;
+  __device__ __forceinline__ explicit BlockSeq(int nblocks) {
+    per = nblocks >> 3;
+    base = (static_cast<int>(blockIdx.x) & 7) * per;
+    first = static_cast<int>(blockIdx.x) >> 3;
+    step = static_cast<int>(gridDim.x) >> 3;
+  }
+};
+
+// Prologue of the halo pipeline: records of the first block, indices of
+// the second.
+template <class T>
+__device__ __forceinline__ void halo_prologue(const StepArgs& a, const BlockSeq& q,
+                                              const T* __restrict__ rec, HaloPipe<T>& pipe) {
+  pipe.load_indices(a, q.base + q.first);
+  pipe.gather_records(rec);
+  if (q.first + q.step < q.per)
+    pipe.load_indices(a, q.base + q.first + q.step);
+  else
+    pipe.none();
 }
 
-// Copy the halo records of block blk into LDS; returns the block's mode
-// (0: staged). The caller synchronises the workgroup before use.
+// Top of block mm: store its halo, start the next block's gathers and the
+// following block's index loads; returns the block's mode (0: staged).
 template <class T>
-__device__ __forceinline__ int stage_halo(const StepArgs& a, int blk, const T* __restrict__ rec,
-                                          T* hrec) {
-  const int mode = a.hmode[blk];
-  if (mode == 0) {
-    const int H = a.hn[blk];
-    const int32_t* __restrict__ hj = a.hj + static_cast<size_t>(blk) * kHaloMax;
-    for (int t = threadIdx.x; t < H; t += kBigBlock) hrec[t] = rec[hj[t]];
+__device__ __forceinline__ int halo_advance(const StepArgs& a, const BlockSeq& q, int mm,
+                                            const T* __restrict__ rec, T* hrec, HaloPipe<T>& pipe) {
+  const int mode = pipe.vmode;
+  pipe.store(hrec);
+  if (mm + q.step < q.per) {
+    pipe.gather_records(rec);
+    if (mm + 2 * q.step < q.per)
+      pipe.load_indices(a, q.base + mm + 2 * q.step);
+    else
+      pipe.none();
   }
   return mode;
 }
@@ -92,11 +165,15 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   Q* tab = reinterpret_cast<Q*>(smem);
   const SplineMeta mF = meta[0], mrho = meta[1];
   for (int q = threadIdx.x; q < w.nwin; q += kBigBlock) tab[q] = coeff[mrho.offset + w.k_lo + q];
-  const int blk = big_block_index(nblocks);
   const T* __restrict__ M = static_cast<const T*>(a.M);
   T* hrec = reinterpret_cast<T*>(smem + window_bytes(w.nwin, sizeof(Q)));
-  const int mode = kStage ? stage_halo<T>(a, blk, M, hrec) : 1;
-  __syncthreads();
+  const BlockSeq seq(nblocks);
+  HaloPipe<T> pipe;
+  if (kStage) halo_prologue<T>(a, seq, M, pipe);
+  for (int mm = seq.first; mm < seq.per; mm += seq.step) {
+  const int blk = seq.base + mm;
+  const int mode = kStage ? halo_advance<T>(a, seq, mm, M, hrec, pipe) : 1;
+  __syncthreads();  // window (first block) and halo in LDS
   const int i = blk * kBigBlock + threadIdx.x;
   double e = 0.0;
   if (i < a.natoms) {
@@ -125,6 +202,8 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     e = static_cast<double>(F.f);
   }
   if (kEnergy) emit_partial<kBigBlock>(e, a.partials, blk);
+  __syncthreads();  // the next block's halo overwrites this one
+  }
 }
 
 template <class R, class Mir, int kEpi, bool kEnergy, bool kWin, int kU, bool kStage>
@@ -155,12 +234,16 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     tB[q] = y;
     if (kEnergy) tE[q] = cp.x;
   }
-  const int blk = big_block_index(nblocks);
   const T* __restrict__ M2 = static_cast<const T*>(a.M2);
   T* hrec = reinterpret_cast<T*>(
       smem + window_bytes(w.nwin, sizeof(Q) + sizeof(V2) + (kEnergy ? sizeof(R) : 0)));
-  const int mode = kStage ? stage_halo<T>(a, blk, M2, hrec) : 1;
-  __syncthreads();
+  const BlockSeq seq(nblocks);
+  HaloPipe<T> pipe;
+  if (kStage) halo_prologue<T>(a, seq, M2, pipe);
+  for (int mm = seq.first; mm < seq.per; mm += seq.step) {
+  const int blk = seq.base + mm;
+  const int mode = kStage ? halo_advance<T>(a, seq, mm, M2, hrec, pipe) : 1;
+  __syncthreads();  // window (first block) and halo in LDS
   const int i = blk * kBigBlock + threadIdx.x;
   double red = 0.0;
   if (i < a.natoms) {
@@ -212,6 +295,8 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     red = kEnergy ? pe : ke;
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial<kBigBlock>(red, a.partials, blk);
+  __syncthreads();  // the next block's halo overwrites this one
+  }
 }
 
 }  // namespace sched

@@ -165,12 +165,13 @@ struct Engine final : EngineBase {
   bool lds = false;
   // Halo staging (k_halo + the kStage passes) for 16-byte gather records.
   static constexpr bool kCanStage = sizeof(typename Mir::T) == 16;
-  static constexpr int kStageCap = 4608;  // halo records per block in LDS (72 KiB)
   bool stage = false;
   int ncu = 0;
+  size_t lds_cu = 160 * 1024;  // LDS of one CU
   int32_t* d_halo_done = nullptr;
   size_t stage_bytes() const { return stage ? static_cast<size_t>(kStageCap) * 16 : 0; }
   int unroll = 4;
+  int lds_grid = 0;   // persistent workgroups of the LDS passes
   int lds_limit = 0;  // per-workgroup dynamic LDS the kernels are allowed
   int nint = 0;       // r-grid intervals
   int budget = 0;     // current window budget
@@ -226,6 +227,11 @@ struct Engine final : EngineBase {
       w_force[e] = window(nint, sizeof(Q) + sizeof(V2) + (e ? sizeof(R) : 0), wb, &b_force[e]);
     }
     unroll = u;
+    // one workgroup per LDS slot of the device: as many as the CU's LDS
+    // holds of the largest pass (at most two of 1024 threads), per CU
+    const size_t per_wg = std::max(b_dens[0], b_force[0]) + stage_bytes() + 1024;
+    const int per_cu = std::max(1, std::min(2, static_cast<int>(lds_cu / per_wg)));
+    lds_grid = std::max(8, std::min(S->nb_force, (std::max(ncu, 8) / 8) * 8 * per_cu));
   }
 
   int init() override {
@@ -260,12 +266,17 @@ struct Engine final : EngineBase {
     nint = m[1].n - 1;
     int rc = 0;
     if ((rc = allow_shapes<false>())) return rc;
+    TDMD_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int lcu = 0;
+    if (hipDeviceGetAttribute(&lcu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
+            hipSuccess &&
+        lcu > 0)
+      lds_cu = static_cast<size_t>(lcu);
     lds = true;
     S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
     if constexpr (kCanStage) {
       if (!S->cfg.disable_halo && lds_limit >= static_cast<int>(kStageCap) * 16 + 16384) {
         if ((rc = allow_shapes<true>())) return rc;
-        TDMD_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         const size_t nbf = static_cast<size_t>(S->nb_force);
         void* p = nullptr;
         const size_t sizes[4] = {nbf * kHaloMax * 4, nbf * 4, nbf * 4, 4};
@@ -353,13 +364,13 @@ struct Engine final : EngineBase {
     const int nb = S->nb_force;
     if constexpr (kCanStage) {
       if (stage) {
-        go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, true>, nb, kBigBlock,
+        go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, true>, lds_grid, kBigBlock,
            b_dens[kEnergy] + stage_bytes(), s, S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy],
            nb);
         return;
       }
     }
-    go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, false>, nb, kBigBlock, b_dens[kEnergy], s,
+    go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, false>, lds_grid, kBigBlock, b_dens[kEnergy], s,
        S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
   }
   template <bool kEnergy>
@@ -381,13 +392,13 @@ struct Engine final : EngineBase {
     const int nb = S->nb_force;
     if constexpr (kCanStage) {
       if (stage) {
-        go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, true>, nb, kBigBlock,
+        go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, true>, lds_grid, kBigBlock,
            b_force[kEnergy] + stage_bytes(), s, S->a, meta(), coeff(), rc_sq(),
            w_force[kEnergy], nb);
         return;
       }
     }
-    go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, false>, nb, kBigBlock, b_force[kEnergy],
+    go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, false>, lds_grid, kBigBlock, b_force[kEnergy],
        s, S->a, meta(), coeff(), rc_sq(), w_force[kEnergy], nb);
   }
 
