@@ -14,6 +14,11 @@
 // The neighbour loop is for_neighbours (sched_kernels.inl), software
 // pipelined in groups of kU slots.
 //
+// kIso (fixed-point mirrors whose three axes share one unit u, e.g. cubic
+// periodic boxes): pair differences stay in units, r^2 / u^2 is tested
+// against rc^2 / u^2 and forces are summed in units and scaled once, which
+// saves three FP64 products per neighbour slot.
+//
 // kStage: the workgroup also copies its block's halo (k_halo) into LDS,
 // behind the table window, and gathers neighbour records from there through
 // the 16-bit rows; a block whose halo did not fit reads the 32-bit rows and
@@ -154,7 +159,22 @@ __host__ __device__ constexpr size_t window_bytes(int nwin, size_t per) {
   return (static_cast<size_t>(nwin) * per + 15) & ~static_cast<size_t>(15);
 }
 
-template <class R, class Mir, bool kEnergy, bool kWin, int kU, bool kStage>
+// Difference of two records (i minus j) and its squared length: in A, or in
+// fixed-point units when kIso.
+template <class Mir, bool kIso, class T>
+__device__ __forceinline__ double pair_delta(const T& a, const T& b, const MirrorParams& mp,
+                                             double* dx, double* dy, double* dz) {
+  if constexpr (kIso) {
+    *dx = static_cast<double>(Mir::wdiff(a.x, b.x));
+    *dy = static_cast<double>(Mir::wdiff(a.y, b.y));
+    *dz = static_cast<double>(Mir::wdiff(a.z, b.z));
+    return *dx * *dx + *dy * *dy + *dz * *dz;
+  } else {
+    return Mir::delta(a, b, mp, dx, dy, dz);
+  }
+}
+
+template <class R, class Mir, bool kEnergy, bool kWin, int kU, bool kStage, bool kIso>
 __global__ void __launch_bounds__(kBigBlock)
 k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
                   const typename Quad<R>::type* __restrict__ coeff, double rc_sq, LdsWindow w,
@@ -181,11 +201,13 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     const MirrorParams& mpi = mp_of<Mir>(a, i);
     const int n = a.cnt[i];
     double rho = 0.0;
+    const double u2 = kIso ? mpi.unit[0] * mpi.unit[0] : 1.0;
+    const double rcut2 = kIso ? rc_sq / u2 : rc_sq;
     auto pair = [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
-      if (in_row && r2 < rc_sq) {
-        const PairR<R> pr = pair_r<R>(r2);
+      const double s2 = pair_delta<Mir, kIso>(mi, mj, mpi, &dx, &dy, &dz);
+      if (in_row && s2 < rcut2) {
+        const PairR<R> pr = pair_r<R>(kIso ? s2 * u2 : s2);
         R t;
         const int idx = spline_locate<R>(mrho, pr.r, &t);
         const int q = idx - w.k_lo;
@@ -206,7 +228,8 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   }
 }
 
-template <class R, class Mir, int kEpi, bool kEnergy, bool kWin, int kU, bool kStage>
+template <class R, class Mir, int kEpi, bool kEnergy, bool kWin, int kU, bool kStage,
+          bool kIso>
 __global__ void __launch_bounds__(kBigBlock)
 k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
                 const typename Quad<R>::type* __restrict__ coeff, double rc_sq, LdsWindow w,
@@ -252,11 +275,14 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     const R fpi = Mir::template w_real<R>(mi);
     const int n = a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
+    const double u = kIso ? mpi.unit[0] : 1.0;
+    const double u2 = u * u;
+    const double rcut2 = kIso ? rc_sq / u2 : rc_sq;
     auto pair = [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
-      if (in_row && r2 < rc_sq) {
-        const PairR<R> pr = pair_r<R>(r2);
+      const double s2 = pair_delta<Mir, kIso>(mi, mj, mpi, &dx, &dy, &dz);
+      if (in_row && s2 < rcut2) {
+        const PairR<R> pr = pair_r<R>(kIso ? s2 * u2 : s2);
         R t;
         const int idx = spline_locate<R>(mphi, pr.r, &t);
         const int q = idx - w.k_lo;
@@ -291,6 +317,11 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
       for_neighbours_staged<T, kU>(a, i, n, hrec, pair);
     else
       for_neighbours<T, kU>(a, i, n, M2, pair);
+    if constexpr (kIso) {  // sums in units
+      fx *= u;
+      fy *= u;
+      fz *= u;
+    }
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
     red = kEnergy ? pe : ke;
   }

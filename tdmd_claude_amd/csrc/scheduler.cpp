@@ -166,6 +166,7 @@ struct Engine final : EngineBase {
   // Halo staging (k_halo + the kStage passes) for 16-byte gather records.
   static constexpr bool kCanStage = sizeof(typename Mir::T) == 16;
   bool stage = false;
+  bool iso = false;  // fixed-point mirror with one unit on all axes (every box)
   int ncu = 0;
   size_t lds_cu = 160 * 1024;  // LDS of one CU
   int32_t* d_halo_done = nullptr;
@@ -193,25 +194,26 @@ struct Engine final : EngineBase {
     return TDMD_OK;
   }
 
-  template <bool kWin, int kU, bool kSt>
+  template <bool kWin, int kU, bool kSt, bool kIs>
   int allow_all() {
     int rc = 0;
     const size_t L = static_cast<size_t>(lds_limit);
-    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin, kU, kSt>, L)) ||
-        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin, kU, kSt>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin, kU, kSt>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin, kU, kSt>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU, kSt>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU, kSt>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU, kSt>, L)))
+    if ((rc = allow_lds(&k_eam_density_lds<R, Mir, false, kWin, kU, kSt, kIs>, L)) ||
+        (rc = allow_lds(&k_eam_density_lds<R, Mir, true, kWin, kU, kSt, kIs>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, true, kWin, kU, kSt, kIs>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kForceOnly, false, kWin, kU, kSt, kIs>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU, kSt, kIs>,
+                        L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU, kSt, kIs>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU, kSt, kIs>, L)))
       return rc;
     return TDMD_OK;
   }
-  template <bool kSt>
+  template <bool kSt, bool kIs>
   int allow_shapes() {
     int rc = 0;
-    if ((rc = allow_all<false, 4, kSt>()) || (rc = allow_all<true, 4, kSt>()) ||
-        (rc = allow_all<false, 8, kSt>()) || (rc = allow_all<true, 8, kSt>()))
+    if ((rc = allow_all<false, 4, kSt, kIs>()) || (rc = allow_all<true, 4, kSt, kIs>()) ||
+        (rc = allow_all<false, 8, kSt, kIs>()) || (rc = allow_all<true, 8, kSt, kIs>()))
       return rc;
     return TDMD_OK;
   }
@@ -265,7 +267,7 @@ struct Engine final : EngineBase {
       return fail(TDMD_E_INVALID, "scheduler: lds_budget below 4096 bytes");
     nint = m[1].n - 1;
     int rc = 0;
-    if ((rc = allow_shapes<false>())) return rc;
+    if ((rc = allow_shapes<false, false>())) return rc;
     TDMD_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     int lcu = 0;
     if (hipDeviceGetAttribute(&lcu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
@@ -275,8 +277,18 @@ struct Engine final : EngineBase {
     lds = true;
     S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
     if constexpr (kCanStage) {
+      iso = true;
+      auto same_unit = [](const MirrorParams& p) {
+        return p.unit[0] == p.unit[1] && p.unit[0] == p.unit[2];
+      };
+      if (S->multibox) {
+        for (const BoxGeo& b : S->h_boxes) iso = iso && same_unit(b.mp);
+      } else {
+        iso = same_unit(S->a.mp);
+      }
+      if (iso && (rc = allow_shapes<false, true>())) return rc;
       if (!S->cfg.disable_halo && lds_limit >= static_cast<int>(kStageCap) * 16 + 16384) {
-        if ((rc = allow_shapes<true>())) return rc;
+        if ((rc = (iso ? allow_shapes<true, true>() : allow_shapes<true, false>()))) return rc;
         const size_t nbf = static_cast<size_t>(S->nb_force);
         void* p = nullptr;
         const size_t sizes[4] = {nbf * kHaloMax * 4, nbf * 4, nbf * 4, 4};
@@ -359,19 +371,26 @@ struct Engine final : EngineBase {
     return TDMD_OK;
   }
 
+  template <bool kEnergy, bool kWin, int kU, bool kSt, bool kIs>
+  void density_go(hipStream_t s) {
+    go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, kSt, kIs>, lds_grid, kBigBlock,
+       b_dens[kEnergy] + (kSt ? stage_bytes() : 0), s, S->a, meta(), coeff(), rc_sq(),
+       w_dens[kEnergy], S->nb_force);
+  }
   template <bool kEnergy, bool kWin, int kU>
   void density_lds(hipStream_t s) {
-    const int nb = S->nb_force;
     if constexpr (kCanStage) {
       if (stage) {
-        go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, true>, lds_grid, kBigBlock,
-           b_dens[kEnergy] + stage_bytes(), s, S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy],
-           nb);
+        iso ? density_go<kEnergy, kWin, kU, true, true>(s)
+            : density_go<kEnergy, kWin, kU, true, false>(s);
+        return;
+      }
+      if (iso) {
+        density_go<kEnergy, kWin, kU, false, true>(s);
         return;
       }
     }
-    go(k_eam_density_lds<R, Mir, kEnergy, kWin, kU, false>, lds_grid, kBigBlock, b_dens[kEnergy], s,
-       S->a, meta(), coeff(), rc_sq(), w_dens[kEnergy], nb);
+    density_go<kEnergy, kWin, kU, false, false>(s);
   }
   template <bool kEnergy>
   void density(hipStream_t s) {
@@ -387,19 +406,26 @@ struct Engine final : EngineBase {
       win ? density_lds<kEnergy, true, 4>(s) : density_lds<kEnergy, false, 4>(s);
   }
 
+  template <int kEpi, bool kEnergy, bool kWin, int kU, bool kSt, bool kIs>
+  void force_go(hipStream_t s) {
+    go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, kSt, kIs>, lds_grid, kBigBlock,
+       b_force[kEnergy] + (kSt ? stage_bytes() : 0), s, S->a, meta(), coeff(), rc_sq(),
+       w_force[kEnergy], S->nb_force);
+  }
   template <int kEpi, bool kEnergy, bool kWin, int kU>
   void force_lds(hipStream_t s) {
-    const int nb = S->nb_force;
     if constexpr (kCanStage) {
       if (stage) {
-        go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, true>, lds_grid, kBigBlock,
-           b_force[kEnergy] + stage_bytes(), s, S->a, meta(), coeff(), rc_sq(),
-           w_force[kEnergy], nb);
+        iso ? force_go<kEpi, kEnergy, kWin, kU, true, true>(s)
+            : force_go<kEpi, kEnergy, kWin, kU, true, false>(s);
+        return;
+      }
+      if (iso) {
+        force_go<kEpi, kEnergy, kWin, kU, false, true>(s);
         return;
       }
     }
-    go(k_eam_force_lds<R, Mir, kEpi, kEnergy, kWin, kU, false>, lds_grid, kBigBlock, b_force[kEnergy],
-       s, S->a, meta(), coeff(), rc_sq(), w_force[kEnergy], nb);
+    force_go<kEpi, kEnergy, kWin, kU, false, false>(s);
   }
 
   template <int kEpi, bool kEnergy>
