@@ -214,6 +214,13 @@ struct StepArgs {
   const int32_t* hn;     // [blocks] halo size
   const int32_t* hmode;  // [blocks] 0: staged in LDS, 1: reads the 32-bit rows
   int stage_cap;         // halo records a pass holds in LDS (0: no staging)
+  // Staged blocks: each density pass lists, in row order, the skin-shell
+  // slots (slot >= cntn) that are inside the cutoff (layout of nbr16, count
+  // cntc); the force pass of the same positions reads the near slots of the
+  // row and then this list.
+  int32_t* cntn;  // [N] near slots of each row (inside the cutoff at build)
+  uint16_t* nbrc16;
+  int32_t* cntc;
   void* M;
   void* M2;
   int32_t* flags;        // [0] rebuild, [1] overflow max, [2] nonfinite,

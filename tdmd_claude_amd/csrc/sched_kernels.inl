@@ -312,12 +312,13 @@ struct Slots16<4> {
   using type = uint2;
 };
 
-// The packed slots k..k+kU-1 of atom i in the 16-bit rows (k a multiple of kU).
+// The packed slots k..k+kU-1 of atom i in 16-bit rows (k a multiple of kU).
 template <int kU>
-__device__ __forceinline__ typename Slots16<kU>::type load_local(const StepArgs& a, int i, int k) {
+__device__ __forceinline__ typename Slots16<kU>::type load_local(const uint16_t* rows, int cap,
+                                                                 int i, int k) {
   static_assert(kU == 4 || kU == 8, "four or eight slots per group");
   using P = typename Slots16<kU>::type;
-  return reinterpret_cast<const P*>(a.nbr16)[ell16(i, k, a.cap) / kU];
+  return reinterpret_cast<const P*>(rows)[ell16(i, k, cap) / kU];
 }
 
 __device__ __forceinline__ void unpack_local(const uint4& v, int (&ln)[8]) {
@@ -338,21 +339,27 @@ __device__ __forceinline__ void unpack_local(const uint2& v, int (&ln)[4]) {
 // for_neighbours over a block whose halo records sit in LDS (`hrec`): the
 // 16-bit rows index the halo, and the records come from LDS. The next
 // group's slots stay packed until the top of the next iteration, so their
-// load is waited for only after this group's pair work.
-template <class T, int kU, class Pair>
-__device__ __forceinline__ void for_neighbours_staged(const StepArgs& a, int i, int n,
-                                                      const T* hrec, Pair&& pair) {
+// load is waited for only after this group's pair work. pair(l, record,
+// in_row, slot). kGuard: slots past n hold no valid index (read slot 0's
+// record instead).
+template <class T, int kU, bool kGuard = false, class Pair>
+__device__ __forceinline__ void for_neighbours_staged(const StepArgs& a, const uint16_t* rows,
+                                                      int i, int n, const T* hrec, Pair&& pair) {
   typename Slots16<kU>::type raw{};
-  if (n > 0) raw = load_local<kU>(a, i, 0);
+  if (n > 0) raw = load_local<kU>(rows, a.cap, i, 0);
   for (int k = 0; k < n; k += kU) {
     int ln[kU];
     unpack_local(raw, ln);
+    if (kGuard) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) ln[u] = k + u < n ? ln[u] : 0;
+    }
     T mj[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) mj[u] = hrec[ln[u]];
-    if (k + kU < n) raw = load_local<kU>(a, i, k + kU);
+    if (k + kU < n) raw = load_local<kU>(rows, a.cap, i, k + kU);
 #pragma unroll
-    for (int u = 0; u < kU; ++u) pair(ln[u], mj[u], k + u < n);
+    for (int u = 0; u < kU; ++u) pair(ln[u], mj[u], k + u < n, k + u);
   }
 }
 
@@ -739,6 +746,7 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
     for (int q = 0; q < m; ++q) nbr[ell(i, nn + q, cap)] = nbr[ell(i, cap - 1 - q, cap)];
     const int n = nn + nf;  // == min(count, cap)
     cnt[i] = n;
+    a.cntn[i] = nn;
     // Pad the row to a multiple of 8 with the atom itself, so unrolled
     // readers can load whole groups of slots (cap is a multiple of 8).
     for (int k = n; k < ((n + 7) & ~7); ++k) nbr[ell(i, k, cap)] = i;

@@ -203,7 +203,14 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     double rho = 0.0;
     const double u2 = kIso ? mpi.unit[0] * mpi.unit[0] : 1.0;
     const double rcut2 = kIso ? rc_sq / u2 : rc_sq;
-    auto pair = [&](int, const T& mj, bool in_row) {
+    // staged blocks: skin-shell slots inside the cutoff are listed for the
+    // force pass (ELL16 layout: the next slot is one element on, or
+    // 64 x 8 - 7 on at the end of a group of eight)
+    const bool cut = kStage && mode == 0;
+    const int nn = cut ? a.cntn[i] : 0;
+    uint16_t* cp = a.nbrc16 + (cut ? ell16(i, 0, a.cap) : 0);
+    int c = 0;
+    auto pair = [&](int l, const T& mj, bool in_row, int slot) {
       double dx, dy, dz;
       const double s2 = pair_delta<Mir, kIso>(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && s2 < rcut2) {
@@ -211,14 +218,21 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
         R t;
         const int idx = spline_locate<R>(mrho, pr.r, &t);
         const int q = idx - w.k_lo;
-        const Q c = (!kWin || q >= 0) ? tab[q] : coeff[mrho.offset + idx];
-        rho += static_cast<double>(cubic_value<R>(c.x, c.y, c.z, c.w, t));
+        const Q cf = (!kWin || q >= 0) ? tab[q] : coeff[mrho.offset + idx];
+        rho += static_cast<double>(cubic_value<R>(cf.x, cf.y, cf.z, cf.w, t));
+        if (cut && slot >= nn) {
+          *cp = static_cast<uint16_t>(l);
+          cp += (c & 7) == 7 ? 505 : 1;
+          ++c;
+        }
       }
     };
-    if (kStage && mode == 0)
-      for_neighbours_staged<T, kU>(a, i, n, hrec, pair);
-    else
-      for_neighbours<T, kU>(a, i, n, M, pair);
+    if (kStage && mode == 0) {
+      for_neighbours_staged<T, kU>(a, a.nbr16, i, n, hrec, pair);
+      a.cntc[i] = c;
+    } else {
+      for_neighbours<T, kU>(a, i, n, M, [&](int j, const T& mj, bool r) { pair(j, mj, r, 0); });
+    }
     const SplineVal<R> F = spline_both<R>(mF, coeff, rho);
     static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
     e = static_cast<double>(F.f);
@@ -273,12 +287,13 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     const T mi = M2[i];
     const MirrorParams& mpi = mp_of<Mir>(a, i);
     const R fpi = Mir::template w_real<R>(mi);
-    const int n = a.cnt[i];
+    const bool cut = kStage && mode == 0;
+    const int n = cut ? a.cntn[i] : a.cnt[i];
     double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
     const double u = kIso ? mpi.unit[0] : 1.0;
     const double u2 = u * u;
     const double rcut2 = kIso ? rc_sq / u2 : rc_sq;
-    auto pair = [&](int, const T& mj, bool in_row) {
+    auto pair = [&](int, const T& mj, bool in_row, int) {
       double dx, dy, dz;
       const double s2 = pair_delta<Mir, kIso>(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && s2 < rcut2) {
@@ -313,10 +328,14 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
         fz = fma(fpair, dz, fz);
       }
     };
-    if (kStage && mode == 0)
-      for_neighbours_staged<T, kU>(a, i, n, hrec, pair);
-    else
-      for_neighbours<T, kU>(a, i, n, M2, pair);
+    if (cut) {
+      // the near slots, then the skin-shell slots the density pass found
+      // inside the cutoff: the row's pairs in row order
+      for_neighbours_staged<T, kU>(a, a.nbr16, i, n, hrec, pair);
+      for_neighbours_staged<T, kU, true>(a, a.nbrc16, i, a.cntc[i], hrec, pair);
+    } else {
+      for_neighbours<T, kU>(a, i, n, M2, [&](int j, const T& mj, bool r) { pair(j, mj, r, 0); });
+    }
     if constexpr (kIso) {  // sums in units
       fx *= u;
       fy *= u;

@@ -291,9 +291,10 @@ struct Engine final : EngineBase {
         if ((rc = (iso ? allow_shapes<true, true>() : allow_shapes<true, false>()))) return rc;
         const size_t nbf = static_cast<size_t>(S->nb_force);
         void* p = nullptr;
-        const size_t sizes[4] = {nbf * kHaloMax * 4, nbf * 4, nbf * 4, 4};
-        void* ptrs[4];
-        for (int k = 0; k < 4; ++k) {
+        const size_t sizes[5] = {nbf * kHaloMax * 4, nbf * 4, nbf * 4, 4,
+                                 static_cast<size_t>(round_up(S->natoms, 64)) * 4};
+        void* ptrs[5];
+        for (int k = 0; k < 5; ++k) {
           if ((rc = tdmd_malloc(&p, sizes[k]))) return rc;
           S->allocs.push_back(p);
           ptrs[k] = p;
@@ -302,6 +303,7 @@ struct Engine final : EngineBase {
         S->a.hn = static_cast<const int32_t*>(ptrs[1]);
         S->a.hmode = static_cast<const int32_t*>(ptrs[2]);
         d_halo_done = static_cast<int32_t*>(ptrs[3]);
+        S->a.cntc = static_cast<int32_t*>(ptrs[4]);
         TDMD_HIP_TRY(hipMemset(ptrs[2], 0xff, sizes[2]));  // no halo yet: not staged
         TDMD_HIP_TRY(hipMemset(d_halo_done, 0, 4));
         S->a.stage_cap = kStageCap;
@@ -624,6 +626,7 @@ int sched_alloc(tdmd_sched* S) {
   int32_t* cnt;
   if ((rc = dalloc(S, &cnt, round_up(N, 64)))) return rc;
   a.cnt = cnt;
+  if ((rc = dalloc(S, &a.cntn, round_up(N, 64)))) return rc;
   // permutation scratch
   PermArgs& t = S->t;
   double** t6[6] = {&t.px, &t.py, &t.pz, &t.vx, &t.vy, &t.vz};
@@ -820,6 +823,7 @@ void sched_free(tdmd_sched* S) {
   for (void* p : S->allocs) (void)hipFree(p);
   if (S->a.nbr) (void)hipFree(const_cast<int32_t*>(S->a.nbr));
   if (S->a.nbr16) (void)hipFree(const_cast<uint16_t*>(S->a.nbr16));
+  if (S->a.nbrc16) (void)hipFree(S->a.nbrc16);
   if (S->tables) eam_tables_destroy(S->tables);
   if (S->stream) (void)hipStreamDestroy(S->stream);
 }
@@ -858,15 +862,19 @@ int first_build(tdmd_sched* S) {
   if (S->a.nbr == nullptr || cap > S->a.cap) {
     if (S->a.nbr) TDMD_HIP_TRY(hipFree(const_cast<int32_t*>(S->a.nbr)));
     if (S->a.nbr16) TDMD_HIP_TRY(hipFree(const_cast<uint16_t*>(S->a.nbr16)));
+    if (S->a.nbrc16) TDMD_HIP_TRY(hipFree(S->a.nbrc16));
     S->a.nbr = nullptr;
     S->a.nbr16 = nullptr;
+    S->a.nbrc16 = nullptr;
     const size_t slots = static_cast<size_t>(round_up(S->natoms, 64)) * cap;
     void* p = nullptr;
     if ((rc = tdmd_malloc(&p, slots * 4))) return rc;
     S->a.nbr = static_cast<int32_t*>(p);
-    if (S->a.stage_cap > 0) {  // 16-bit rows of the staged passes
+    if (S->a.stage_cap > 0) {  // 16-bit rows of the staged passes, full and cut
       if ((rc = tdmd_malloc(&p, slots * 2))) return rc;
       S->a.nbr16 = static_cast<uint16_t*>(p);
+      if ((rc = tdmd_malloc(&p, slots * 2))) return rc;
+      S->a.nbrc16 = static_cast<uint16_t*>(p);
     }
     S->a.cap = cap;
     for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
