@@ -73,14 +73,19 @@ def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[fl
     return wall, ms.value / 1000.0
 
 
-def bytes_per_atom(precision: str, mirror_fp64: bool, nbar: float) -> dict:
-    """Algorithmic HBM bytes per atom of the two steady-state passes
-    (each per-atom array touched once per pass, neighbour indices read once
-    per pass, neighbour records served from cache)."""
+def bytes_per_atom(precision: str, mirror_fp64: bool, nbar: float, staged: float = 0.0,
+                   halo: float = 0.0) -> dict:
+    """Algorithmic HBM bytes per atom of the two steady-state passes: each
+    per-atom array touched once per pass, the neighbour rows read once per
+    pass, neighbour records served from cache (L2, or LDS for staged
+    blocks). Rows are 4 bytes a slot, or 2 bytes a slot plus the 4-byte halo
+    list (`halo` atoms per atom) for the fraction `staged` of blocks whose
+    halo is staged in LDS."""
     rec = 32 if (precision == "fp64" or mirror_fp64) else 16  # M / M2 record
-    density = rec + 4 + 4 * nbar + rec  # read M_i, count, indices; write M2_i
-    # force pass: read M2_i, count, type, x, v, x_build, indices; write x, v, M_i
-    force = rec + 4 + 4 + 24 + 24 + 24 + 4 * nbar + 24 + 24 + rec
+    rows = staged * (2 * nbar + 4 * halo) + (1 - staged) * 4 * nbar
+    density = rec + 4 + rows + rec  # read M_i, count, rows; write M2_i
+    # force pass: read M2_i, count, type, x, v, x_build, rows; write x, v, M_i
+    force = rec + 4 + 4 + 24 + 24 + 24 + rows + 24 + 24 + rec
     return dict(density=density, force=force, step=density + force)
 
 
@@ -339,8 +344,11 @@ def main() -> int:
     nve.run_until(0)
     nbar = nve.thermo()["mean_nbr"]
     t_dens, t_force = nve.time_passes(30)
+    halo = nve.halo_info()
     nve.close()
-    bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar)
+    staged = halo["staged"] / halo["blocks"] if halo["blocks"] else 0.0
+    halo_pa = halo["halo_atoms"] / N if halo["blocks"] else 0.0
+    bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar, staged, halo_pa)
     traffic = measured_traffic(N, args.mirror_fp64)
     achieved = N * bpa["force"] / (t_force * 1e-3) / 1e9
     log(f"passes: density {t_dens:.4f} ms, force {t_force:.4f} ms; force pass "
@@ -456,6 +464,8 @@ def main() -> int:
             "bytes_per_atom": round(bpa["force"], 1),
             "avg_ms": round(t_force, 5),
             "density_pass_ms": round(t_dens, 5),
+            "halo_staged_blocks": f"{halo['staged']}/{halo['blocks']}",
+            "halo_atoms_per_atom": round(halo_pa, 3),
         },
         "cpu_baseline": cpu,
         "extras": extras,
