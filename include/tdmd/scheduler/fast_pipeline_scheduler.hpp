@@ -131,6 +131,13 @@ class FastPipelineScheduler {
     TDMD_HIP_CALL(tdmd_sched_thermo(s_, out.data()));
     return out;
   }
+  // Device FP64 observables: energies, T, chain and conserved energy,
+  // virial, volume, pressure (bar), neighbour counts.
+  tdmd_observables observables() {
+    tdmd_observables o{};
+    TDMD_HIP_CALL(tdmd_sched_observables(s_, &o));
+    return o;
+  }
   hipStream_t stream() const {
     void* s = nullptr;
     TDMD_HIP_CALL(tdmd_sched_stream(s_, &s));

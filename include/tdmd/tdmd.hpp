@@ -9,6 +9,8 @@
 #include "domain/device_cell_list.hpp"
 #include "integrator/device_nose_hoover.hpp"
 #include "integrator/device_velocity_verlet.hpp"
+#include "io/dump.hpp"
+#include "io/thermo.hpp"
 #include "neighbors/device_neighbor_list.hpp"
 #include "potentials/device_eam.hpp"
 #include "potentials/device_morse.hpp"

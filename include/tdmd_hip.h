@@ -334,6 +334,24 @@ int tdmd_sched_interaction_cutoff(const tdmd_sched* s, double* rc);
  * out[0] PE, [1] KE, [2] NHC chain energy, [3] T (3N dof), [4] max
  * neighbour count, [5] mean neighbour count. Runs one energy pass. */
 int tdmd_sched_thermo(tdmd_sched* s, double out[6]);
+/* Observables at the current step (no reference counterpart; the thermo
+ * and pressure a driver reports, tdmd_main.cpp:97-122, computed on the
+ * device in FP64). One energy pass. virial W = sum over pairs of r_ij .
+ * f_ij (eV); pressure P = (2 KE + W) / (3 V) in bar; T from 3N degrees of
+ * freedom as the reference's thermo line; conserved = PE + KE + chain
+ * energy (NVT) or PE + KE (NVE). Batches: sums over the replicas, V the
+ * summed box volumes. */
+typedef struct tdmd_observables {
+  int64_t step;
+  int64_t natoms;
+  double pe, ke, te, temp;
+  double nhc_energy, conserved;
+  double virial, volume, pressure;
+  int32_t max_neighbors;
+  int32_t pad;
+  double mean_neighbors;
+} tdmd_observables;
+int tdmd_sched_observables(tdmd_sched* s, tdmd_observables* out);
 /* Timing window (no reference counterpart; measurement aid): runs `steps`
  * NVE steps without rebuild tests, with HIP events around each pass of the
  * steady-state step, and returns the mean durations: ms_out[0] = EAM

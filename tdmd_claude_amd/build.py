@@ -60,6 +60,7 @@ def _compile(src: str) -> pathlib.Path:
 
 
 CPP_TEST = ROOT / "tests" / "cpp" / "dropin_test.cpp"
+CPP_IO_TEST = ROOT / "tests" / "cpp" / "io_test.cpp"
 CPP_BIN = ROOT / "tests" / "cpp" / "_bin"
 
 
@@ -79,6 +80,18 @@ def build_cpp_tests(verbose: bool = True) -> list[pathlib.Path]:
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"C++ test build failed:\n{' '.join(cmd)}\n{res.stdout}\n"
+                               f"{res.stderr}")
+        out.append(exe)
+        if verbose:
+            print(f"built {exe}")
+        # host-only io checks: no device code, no library
+        exe = CPP_BIN / f"io_test_{prec.lower()}"
+        cmd = [CXX, "-std=c++20", "-O2", "-Wall", f"-DTDMD_PRECISION_{prec}",
+               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", f"-I{ROOT / 'include'}",
+               str(CPP_IO_TEST), "-o", str(exe)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"C++ io test build failed:\n{' '.join(cmd)}\n{res.stdout}\n"
                                f"{res.stderr}")
         out.append(exe)
         if verbose:

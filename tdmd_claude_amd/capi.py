@@ -89,6 +89,27 @@ class SchedConfig(C.Structure):
     ]
 
 
+class Observables(C.Structure):
+    """tdmd_observables."""
+
+    _fields_ = [
+        ("step", C.c_int64),
+        ("natoms", C.c_int64),
+        ("pe", C.c_double),
+        ("ke", C.c_double),
+        ("te", C.c_double),
+        ("temp", C.c_double),
+        ("nhc_energy", C.c_double),
+        ("conserved", C.c_double),
+        ("virial", C.c_double),
+        ("volume", C.c_double),
+        ("pressure", C.c_double),
+        ("max_neighbors", C.c_int32),
+        ("pad", C.c_int32),
+        ("mean_neighbors", C.c_double),
+    ]
+
+
 _VP = C.c_void_p
 _I32 = C.c_int32
 _I64 = C.c_int64
@@ -170,6 +191,7 @@ _PROTOS = {
     "tdmd_sched_current_step": (C.c_int, [_VP, _PI32]),
     "tdmd_sched_interaction_cutoff": (C.c_int, [_VP, _PD]),
     "tdmd_sched_thermo": (C.c_int, [_VP, _PD]),
+    "tdmd_sched_observables": (C.c_int, [_VP, C.POINTER(Observables)]),
     "tdmd_sched_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
     "tdmd_sched_halo_info": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
     "tdmd_sched_time_passes": (C.c_int, [_VP, _I32, C.POINTER(C.c_float)]),

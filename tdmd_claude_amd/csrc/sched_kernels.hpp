@@ -226,6 +226,7 @@ struct StepArgs {
   int32_t* flags;        // [0] rebuild, [1] overflow max, [2] nonfinite,
                          // [3] barrier timeout, [4] halos stale
   double* partials;      // per-block FP64 partial sums
+  double* partials_w;    // per-block virial partial sums (energy passes)
   double* nhc;           // device NHC state block
   // batches of boxes (ManyBoxes mirrors only)
   const BoxGeo* boxes;   // [nrep]

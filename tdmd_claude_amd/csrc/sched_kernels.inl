@@ -415,7 +415,7 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
   const int blk = xcd_block(a.nblocks);
   const int i = blk * kBlock + threadIdx.x;
   const T* __restrict__ M2 = static_cast<const T*>(a.M2);
-  double red = 0.0;
+  double red = 0.0, vir = 0.0;
   if (i < a.natoms) {
     const T mi = M2[i];
     const MirrorParams& mpi = mp_of<Mir>(a, i);
@@ -442,6 +442,7 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
         const R drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
         const R fpj = Mir::template w_real<R>(mj);
         const double fpair = static_cast<double>(-(dphi + fpi * drho_j + fpj * drho_i) * pr.rinv);
+        if (kEnergy) vir += 0.5 * fpair * r2;
         fx = fma(fpair, dx, fx);
         fy = fma(fpair, dy, fy);
         fz = fma(fpair, dz, fz);
@@ -451,6 +452,7 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
     red = kEnergy ? pe : ke;
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
+  if (kEnergy) emit_partial(vir, a.partials_w, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,7 +467,7 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
   const int blk = xcd_block(a.nblocks);
   const int i = blk * kBlock + threadIdx.x;
   const T* __restrict__ M = static_cast<const T*>(a.M);
-  double red = 0.0;
+  double red = 0.0, vir = 0.0;
   if (i < a.natoms) {
     const T mi = M[i];
     const MirrorParams& mpi = mp_of<Mir>(a, i);
@@ -478,7 +480,10 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
         const PairR<R> pr = pair_r<R>(r2);
         R e;
         const double fpair = static_cast<double>(morse_fpair<R>(p, pr, &e));
-        if (kEnergy) pe += 0.5 * static_cast<double>(e);
+        if (kEnergy) {
+          pe += 0.5 * static_cast<double>(e);
+          vir += 0.5 * fpair * r2;
+        }
         fx = fma(fpair, dx, fx);
         fy = fma(fpair, dy, fy);
         fz = fma(fpair, dz, fz);
@@ -488,6 +493,7 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
     red = kEnergy ? pe : ke;
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
+  if (kEnergy) emit_partial(vir, a.partials_w, blk);
 }
 
 // ---------------------------------------------------------------------------

@@ -282,14 +282,14 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   const int mode = kStage ? halo_advance<T>(a, seq, mm, M2, hrec, pipe) : 1;
   __syncthreads();  // window (first block) and halo in LDS
   const int i = blk * kBigBlock + threadIdx.x;
-  double red = 0.0;
+  double red = 0.0, wsum = 0.0;
   if (i < a.natoms) {
     const T mi = M2[i];
     const MirrorParams& mpi = mp_of<Mir>(a, i);
     const R fpi = Mir::template w_real<R>(mi);
     const bool cut = kStage && mode == 0;
     const int n = cut ? a.cntn[i] : a.cnt[i];
-    double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, vir = 0.0;
     const double u = kIso ? mpi.unit[0] : 1.0;
     const double u2 = u * u;
     const double rcut2 = kIso ? rc_sq / u2 : rc_sq;
@@ -323,6 +323,7 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
         if (kEnergy) pe += 0.5 * static_cast<double>(ephi);
         const R fpj = Mir::template w_real<R>(mj);
         const double fpair = static_cast<double>(-(dphi + fpi * drho + fpj * drho) * pr.rinv);
+        if (kEnergy) vir += 0.5 * fpair * (kIso ? s2 * u2 : s2);
         fx = fma(fpair, dx, fx);
         fy = fma(fpair, dy, fy);
         fz = fma(fpair, dz, fz);
@@ -343,8 +344,10 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     }
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
     red = kEnergy ? pe : ke;
+    wsum = vir;
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial<kBigBlock>(red, a.partials, blk);
+  if (kEnergy) emit_partial<kBigBlock>(wsum, a.partials_w, blk);
   __syncthreads();  // the next block's halo overwrites this one
   }
 }

@@ -515,6 +515,7 @@ struct Engine final : EngineBase {
     }
     force<kForceOnly, true>(s);
     launch_sum_partials(S->a.partials, nb, d_out + 1, 0, s);
+    launch_sum_partials(S->a.partials_w, nb, d_out + 3, 0, s);
     k_ke_partials<R><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a);
     launch_sum_partials(S->a.partials, grid_for(S->natoms), d_out + 2, 0, s);
     TDMD_LAUNCH_CHECK();
@@ -622,6 +623,7 @@ int sched_alloc(tdmd_sched* S) {
   a.M2 = M2;
   if ((rc = dalloc(S, &a.flags, 8))) return rc;
   if ((rc = dalloc(S, &a.partials, S->nblocks + 64))) return rc;
+  if ((rc = dalloc(S, &a.partials_w, S->nblocks + 64))) return rc;
   if ((rc = dalloc(S, &a.nhc, 64))) return rc;
   int32_t* cnt;
   if ((rc = dalloc(S, &cnt, round_up(N, 64)))) return rc;
@@ -1370,6 +1372,37 @@ int tdmd_sched_halo_info(tdmd_sched* S, int64_t out[4]) {
 int tdmd_sched_stream(const tdmd_sched* S, void** stream) {
   if (!S || !stream) return fail(TDMD_E_INVALID, "sched_stream: null");
   *stream = S->stream;
+  return TDMD_OK;
+}
+
+int tdmd_sched_observables(tdmd_sched* S, tdmd_observables* o) {
+  if (!S || !o) return fail(TDMD_E_INVALID, "sched_observables: null");
+  if (!S->uploaded) return fail(TDMD_E_STATE, "sched_observables: upload() first");
+  double th[6];
+  int rc = tdmd_sched_thermo(S, th);
+  if (rc) return rc;
+  double sc[4];
+  TDMD_HIP_TRY(hipMemcpy(sc, S->d_scalars, 32, hipMemcpyDeviceToHost));
+  double vol = 0.0;
+  if (S->multibox) {
+    for (const auto& b : S->h_boxes) vol += b.mp.L[0] * b.mp.L[1] * b.mp.L[2];
+  } else {
+    vol = S->a.mp.L[0] * S->a.mp.L[1] * S->a.mp.L[2];
+  }
+  o->step = S->current_step;
+  o->natoms = S->natoms;
+  o->pe = th[0];
+  o->ke = th[1];
+  o->te = th[0] + th[1];
+  o->temp = th[3];
+  o->nhc_energy = th[2];
+  o->conserved = o->te + th[2];
+  o->virial = sc[3];
+  o->volume = vol;
+  // P = (2 KE + W) / (3 V), eV/A^3 -> bar (1 eV/A^3 = 1.602176634e6 bar)
+  o->pressure = (2.0 * th[1] + sc[3]) / (3.0 * vol) * 1.602176634e6;
+  o->max_neighbors = static_cast<int32_t>(th[4]);
+  o->mean_neighbors = th[5];
   return TDMD_OK;
 }
 

@@ -168,6 +168,13 @@ class FastPipelineScheduler:
         return dict(pe=out[0], ke=out[1], nhc=out[2], temp=out[3], max_nbr=out[4],
                     mean_nbr=out[5])
 
+    def observables(self) -> dict:
+        """PE, KE, total, T, chain and conserved energy, virial (eV), volume
+        (A^3), pressure (bar), neighbour counts at the current step."""
+        o = capi.Observables()
+        capi.call("tdmd_sched_observables", self._h, C.byref(o))
+        return {name: getattr(o, name) for name, _ in capi.Observables._fields_ if name != "pad"}
+
     def time_passes(self, steps: int = 20) -> tuple[float, float]:
         """Mean ms of the two passes of a steady-state NVE step (advances state)."""
         out = (C.c_float * 2)()

@@ -6,7 +6,7 @@
 // (fast_pipeline_scheduler.cu:101-166,195-232). Built twice, once per
 // precision macro. Prints one line per check; exit status 0 = all passed.
 //
-// usage: dropin_test <path to a setfl file>
+// usage: dropin_test <path to a setfl file> [dump file to write]
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -22,6 +22,7 @@ using namespace tdmd;
 namespace {
 
 int g_failures = 0;
+std::string g_dump_path = "dropin_test.dump";
 
 void report(const char* name, bool ok, const std::string& detail) {
   std::printf("%s %s: %s\n", ok ? "PASS" : "FAIL", name, detail.c_str());
@@ -365,6 +366,32 @@ void check_eam(const System& s, const std::string& setfl) {
              std::fabs(th0[0] - e[0]) <= 1e-6 * std::fabs(e[0]),
          "|dE|/N " + std::to_string(de) + " eV, PE0 " + std::to_string(th0[0]) +
              " vs DeviceEam " + std::to_string(e[0]));
+
+  // driver outputs: observables -> thermo line; download -> force dump,
+  // read back with the reference's reader semantics (sorted by id)
+  const tdmd_observables ob = sch.observables();
+  std::vector<PositionVec> p1(N);
+  std::vector<VelocityVec> v1(N);
+  std::vector<ForceVec> f1(N);
+  std::vector<i32> t1(N), id1(N);
+  sch.download(p1.data(), v1.data(), f1.data(), t1.data(), id1.data(), N);
+  io::write_lammps_force_dump(g_dump_path, ob.step, s.box, N, id1.data(), t1.data(), p1.data(),
+                              f1.data());
+  const auto back = io::read_lammps_dump(g_dump_path);
+  bool same = static_cast<int>(back.size()) == N;
+  for (int k = 0; same && k < N; ++k) {
+    const auto& b = back[static_cast<size_t>(k)];
+    const int q = static_cast<int>(std::find(id1.begin(), id1.end(), b.id) - id1.begin());
+    same = q < N && b.type == t1[q] && b.pos.x == p1[q].x && b.pos.y == p1[q].y &&
+           b.pos.z == p1[q].z && b.force.x == f1[q].x && b.force.z == f1[q].z &&
+           (k == 0 || back[static_cast<size_t>(k) - 1].id < b.id);
+  }
+  const std::string line = io::thermo_line(ob.step, ob.pe, ob.ke, ob.natoms);
+  const bool th_ok = ob.step == 200 && std::fabs(ob.pe - th1[0]) <= 1e-9 * std::fabs(th1[0]) &&
+                     std::fabs(ob.ke - th1[1]) <= 1e-9 * th1[1] &&
+                     line.rfind("Step      200  PE ", 0) == 0 && std::isfinite(ob.pressure);
+  report("observables, thermo line and force dump round trip", same && th_ok,
+         line + "; P " + std::to_string(ob.pressure) + " bar");
 }
 
 void check_nhc_reductions(const System& s) {
@@ -414,6 +441,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "usage: %s <setfl file>\n", argv[0]);
     return 2;
   }
+  if (argc > 2) g_dump_path = argv[2];
   std::printf("precision: %s\n", kMixed ? "mixed" : "fp64");
   std::fflush(stdout);
   const System s = fcc(4, 3.615, 0.05, 4.0, 42);  // 256 atoms, box 14.46 A
