@@ -9,10 +9,12 @@
 #include "domain/device_cell_list.hpp"
 #include "integrator/device_nose_hoover.hpp"
 #include "integrator/device_velocity_verlet.hpp"
+#include "integrator/device_velocity_verlet_zone.hpp"
 #include "io/dump.hpp"
 #include "io/thermo.hpp"
 #include "neighbors/device_neighbor_list.hpp"
 #include "potentials/device_eam.hpp"
 #include "potentials/device_morse.hpp"
+#include "potentials/device_morse_zone.hpp"
 #include "potentials/setfl_potential.hpp"
 #include "scheduler/fast_pipeline_scheduler.hpp"

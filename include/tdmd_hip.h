@@ -144,6 +144,17 @@ int tdmd_morse_force(int32_t precision, const double* d_pos, void* d_forces,
                      const int32_t* d_counts, int32_t natoms,
                      const tdmd_box* box, const tdmd_morse_params* params,
                      double* d_energy, void* d_work, void* stream);
+/* Zone pass (compute_morse_gpu_zone, src/potentials/device_morse_zone.cuh:
+ * 14-22): atoms [first_atom, first_atom + atom_count) over the whole CSR
+ * list and position array; forces are ADDED to d_forces (the caller zeroes
+ * the zone, device_morse_zone.cu:110-112); *d_energy is added to. d_work: as
+ * tdmd_morse_force, one double per 256 zone atoms. */
+int tdmd_morse_force_zone(int32_t precision, const double* d_pos, void* d_forces,
+                          const int32_t* d_neighbors, const int32_t* d_offsets,
+                          const int32_t* d_counts, int32_t first_atom,
+                          int32_t atom_count, const tdmd_box* box,
+                          const tdmd_morse_params* params, double* d_energy,
+                          void* d_work, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* FD-Hermite spline (SplineTable::build_splines,                      */
@@ -217,6 +228,17 @@ int tdmd_fused_kick_drift(int32_t precision, int32_t full_kick, double* d_vel,
                           void* stream);
 int tdmd_zero_forces(int32_t precision, void* d_forces, int32_t natoms,
                      void* stream);
+/* Zone variants (src/integrator/device_velocity_verlet_zone.cuh:12-26):
+ * the kernels above over atoms [first_atom, first_atom + atom_count). */
+int tdmd_half_kick_zone(int32_t precision, double* d_vel, const void* d_forces,
+                        const int32_t* d_types, const void* d_masses,
+                        int32_t first_atom, int32_t atom_count, double dt,
+                        void* stream);
+int tdmd_drift_zone(double* d_pos, const double* d_vel, int32_t first_atom,
+                    int32_t atom_count, double dt, const tdmd_box* box,
+                    void* stream);
+int tdmd_zero_forces_zone(int32_t precision, void* d_forces, int32_t first_atom,
+                          int32_t atom_count, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Nose-Hoover support (src/integrator/device_nose_hoover.cuh:26-88).  */

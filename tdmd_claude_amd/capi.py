@@ -166,6 +166,11 @@ _PROTOS = {
     "tdmd_fused_kick_drift": (C.c_int, [_I32, _I32, _VP, _VP, _VP, _VP, _VP, _I32, _D,
                                         C.POINTER(Box), _VP]),
     "tdmd_zero_forces": (C.c_int, [_I32, _VP, _I32, _VP]),
+    "tdmd_morse_force_zone": (C.c_int, [_I32, _VP, _VP, _VP, _VP, _VP, _I32, _I32,
+                                        C.POINTER(Box), C.POINTER(MorseParams), _VP, _VP, _VP]),
+    "tdmd_half_kick_zone": (C.c_int, [_I32, _VP, _VP, _VP, _VP, _I32, _I32, _D, _VP]),
+    "tdmd_drift_zone": (C.c_int, [_VP, _VP, _I32, _I32, _D, C.POINTER(Box), _VP]),
+    "tdmd_zero_forces_zone": (C.c_int, [_I32, _VP, _I32, _I32, _VP]),
     "tdmd_reduce_workspace_bytes": (C.c_int, [_I64, C.POINTER(_SZ)]),
     "tdmd_compute_ke": (C.c_int, [_I32, _VP, _VP, _VP, _I32, _VP, _VP, _VP]),
     "tdmd_scale_velocities": (C.c_int, [_VP, _I32, _D, _VP]),

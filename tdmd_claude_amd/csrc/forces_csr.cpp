@@ -43,15 +43,18 @@ __device__ __forceinline__ void emit_energy(double e, double* __restrict__ parti
   }
 }
 
-template <class R, bool kEnergy>
+// Atoms [first, first + count); kAccum adds the forces to what `forces`
+// holds (the zone pass, device_morse_zone.cu:29-117) instead of storing them.
+template <class R, bool kEnergy, bool kAccum>
 __global__ void __launch_bounds__(kBlock)
 morse_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
                  const int32_t* __restrict__ nbrs, const int32_t* __restrict__ offsets,
-                 const int32_t* __restrict__ counts, int natoms, DevBox box, MorseCoef p,
-                 double* __restrict__ partials, double* __restrict__ d_energy) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                 const int32_t* __restrict__ counts, int first, int count, DevBox box,
+                 MorseCoef p, double* __restrict__ partials, double* __restrict__ d_energy) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = first + t;
   double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-  if (i < natoms) {
+  if (t < count) {
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const int off = offsets[i], cnt = counts[i];
     for (int k = 0; k < cnt; ++k) {
@@ -67,9 +70,15 @@ morse_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
         fz = fma(fpair, dz, fz);
       }
     }
-    forces[3 * i] = static_cast<R>(fx);
-    forces[3 * i + 1] = static_cast<R>(fy);
-    forces[3 * i + 2] = static_cast<R>(fz);
+    if (kAccum) {
+      forces[3 * i] += static_cast<R>(fx);
+      forces[3 * i + 1] += static_cast<R>(fy);
+      forces[3 * i + 2] += static_cast<R>(fz);
+    } else {
+      forces[3 * i] = static_cast<R>(fx);
+      forces[3 * i + 1] = static_cast<R>(fy);
+      forces[3 * i + 2] = static_cast<R>(fz);
+    }
   }
   if (kEnergy) emit_energy(pe, partials, d_energy);
 }
@@ -180,21 +189,23 @@ eam_force_csr_kernel(const double* __restrict__ pos, R* __restrict__ forces,
   if (kEnergy) emit_energy(pe, partials, d_energy);
 }
 
-template <class R>
+template <class R, bool kAccum>
 int morse_launch(const double* pos, void* forces, const int32_t* nbrs, const int32_t* offsets,
-                 const int32_t* counts, int natoms, const DevBox& box,
+                 const int32_t* counts, int first, int count, const DevBox& box,
                  const tdmd_morse_params& p, double* d_energy, void* work, hipStream_t stream) {
-  const int grid = grid_for(natoms);
+  const int grid = grid_for(count);
   double* partials = static_cast<double*>(work);
   const MorseCoef c{p.d, p.alpha, p.r0, p.rc_sq};
   if (d_energy) {
-    morse_csr_kernel<R, true><<<grid, kBlock, 0, stream>>>(
-        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, c, partials, d_energy);
+    morse_csr_kernel<R, true, kAccum><<<grid, kBlock, 0, stream>>>(
+        pos, static_cast<R*>(forces), nbrs, offsets, counts, first, count, box, c, partials,
+        d_energy);
     TDMD_LAUNCH_CHECK();
     if (partials) return launch_sum_partials(partials, grid, d_energy, 1, stream);
   } else {
-    morse_csr_kernel<R, false><<<grid, kBlock, 0, stream>>>(
-        pos, static_cast<R*>(forces), nbrs, offsets, counts, natoms, box, c, nullptr, nullptr);
+    morse_csr_kernel<R, false, kAccum><<<grid, kBlock, 0, stream>>>(
+        pos, static_cast<R*>(forces), nbrs, offsets, counts, first, count, box, c, nullptr,
+        nullptr);
     TDMD_LAUNCH_CHECK();
   }
   return TDMD_OK;
@@ -260,12 +271,35 @@ int tdmd_morse_force(int32_t precision, const double* d_pos, void* d_forces,
     return fail(TDMD_E_INVALID, "morse_force: null pointer");
   const DevBox db = make_devbox(*box);
   if (precision == TDMD_MIXED)
-    return morse_launch<float>(d_pos, d_forces, d_neighbors, d_offsets, d_counts, natoms, db,
-                               *params, d_energy, d_work, as_stream(stream));
+    return morse_launch<float, false>(d_pos, d_forces, d_neighbors, d_offsets, d_counts, 0,
+                                      natoms, db, *params, d_energy, d_work, as_stream(stream));
   if (precision == TDMD_FP64)
-    return morse_launch<double>(d_pos, d_forces, d_neighbors, d_offsets, d_counts, natoms, db,
-                                *params, d_energy, d_work, as_stream(stream));
+    return morse_launch<double, false>(d_pos, d_forces, d_neighbors, d_offsets, d_counts, 0,
+                                       natoms, db, *params, d_energy, d_work,
+                                       as_stream(stream));
   return fail(TDMD_E_INVALID, "morse_force: bad precision");
+}
+
+int tdmd_morse_force_zone(int32_t precision, const double* d_pos, void* d_forces,
+                          const int32_t* d_neighbors, const int32_t* d_offsets,
+                          const int32_t* d_counts, int32_t first_atom, int32_t atom_count,
+                          const tdmd_box* box, const tdmd_morse_params* params,
+                          double* d_energy, void* d_work, void* stream) {
+  if (!box || !params || first_atom < 0 || atom_count < 0)
+    return fail(TDMD_E_INVALID, "morse_force_zone: bad arguments");
+  if (atom_count == 0) return TDMD_OK;  // device_morse_zone.cu:126
+  if (!d_pos || !d_forces || !d_offsets || !d_counts)
+    return fail(TDMD_E_INVALID, "morse_force_zone: null pointer");
+  const DevBox db = make_devbox(*box);
+  if (precision == TDMD_MIXED)
+    return morse_launch<float, true>(d_pos, d_forces, d_neighbors, d_offsets, d_counts,
+                                     first_atom, atom_count, db, *params, d_energy, d_work,
+                                     as_stream(stream));
+  if (precision == TDMD_FP64)
+    return morse_launch<double, true>(d_pos, d_forces, d_neighbors, d_offsets, d_counts,
+                                      first_atom, atom_count, db, *params, d_energy, d_work,
+                                      as_stream(stream));
+  return fail(TDMD_E_INVALID, "morse_force_zone: bad precision");
 }
 
 int tdmd_eam_compute(const tdmd_eam_tables* t, const double* d_pos, void* d_forces,

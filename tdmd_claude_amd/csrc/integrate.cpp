@@ -263,4 +263,41 @@ double tdmd_nhc_half_step(int32_t M, double* eta, double* eta_v, double* eta_m, 
   return nhc_half_step(M, eta, eta_v, eta_m, t_target, t_period, dt, n_dof, ke);
 }
 
+// Zone variants (device_velocity_verlet_zone.cu:14-118): the same kernels
+// over atoms [first_atom, first_atom + atom_count) of the arrays.
+static size_t force_bytes(int32_t precision) { return precision == TDMD_MIXED ? 4 : 8; }
+
+int tdmd_half_kick_zone(int32_t precision, double* d_vel, const void* d_forces,
+                        const int32_t* d_types, const void* d_masses, int32_t first_atom,
+                        int32_t atom_count, double dt, void* stream) {
+  if (first_atom < 0 || atom_count < 0) return fail(TDMD_E_INVALID, "half_kick_zone: bad range");
+  if (atom_count == 0) return TDMD_OK;
+  if (!d_vel || !d_forces || !d_types || !d_masses)
+    return fail(TDMD_E_INVALID, "half_kick_zone: null pointer");
+  const size_t f0 = 3 * static_cast<size_t>(first_atom);
+  return tdmd_half_kick(precision, d_vel + f0,
+                        static_cast<const char*>(d_forces) + f0 * force_bytes(precision),
+                        d_types + first_atom, d_masses, atom_count, dt, stream);
+}
+
+int tdmd_drift_zone(double* d_pos, const double* d_vel, int32_t first_atom, int32_t atom_count,
+                    double dt, const tdmd_box* box, void* stream) {
+  if (!box || first_atom < 0 || atom_count < 0)
+    return fail(TDMD_E_INVALID, "drift_zone: bad arguments");
+  if (atom_count == 0) return TDMD_OK;
+  if (!d_pos || !d_vel) return fail(TDMD_E_INVALID, "drift_zone: null pointer");
+  const size_t f0 = 3 * static_cast<size_t>(first_atom);
+  return tdmd_drift(d_pos + f0, d_vel + f0, atom_count, dt, box, stream);
+}
+
+int tdmd_zero_forces_zone(int32_t precision, void* d_forces, int32_t first_atom,
+                          int32_t atom_count, void* stream) {
+  if (first_atom < 0 || atom_count < 0) return fail(TDMD_E_INVALID, "zero_forces_zone: bad range");
+  if (atom_count == 0) return TDMD_OK;
+  if (!d_forces) return fail(TDMD_E_INVALID, "zero_forces_zone: null pointer");
+  const size_t f0 = 3 * static_cast<size_t>(first_atom);
+  return tdmd_zero_forces(precision, static_cast<char*>(d_forces) + f0 * force_bytes(precision),
+                          atom_count, stream);
+}
+
 }  // extern "C"
