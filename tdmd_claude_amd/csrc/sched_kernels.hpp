@@ -57,6 +57,7 @@ __device__ __forceinline__ double min_image_sel(double d, double L, double hL, i
 struct MirrorF64 {
   using T = double4;
   static constexpr bool kMB = false;
+  static constexpr bool kFixed = false;
   __device__ __forceinline__ static T encode(const MirrorParams&, double x, double y, double z,
                                              double w) {
     return make_double4(x, y, z, w);
@@ -91,6 +92,9 @@ struct MirrorF64 {
 struct MirrorFixed {
   using T = int4;
   static constexpr bool kMB = false;
+  // build positions (skin test) kept as the 32-bit coordinates, in the
+  // first 4 bytes per atom of bx/by/bz
+  static constexpr bool kFixed = true;
   // Coordinate in units, reduced mod 2^32 (a periodic image on a periodic
   // axis; an open axis stays in range, see representable()).
   __device__ __forceinline__ static int enc1(double x, double lo, double q) {

@@ -75,10 +75,20 @@ __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vsca
   a.px[i] = x;
   a.py[i] = y;
   a.pz[i] = z;
-  static_cast<typename Mir::T*>(a.M)[i] = encode_type<Mir>(mp, x, y, z, ti);
-  const double dx = min_image(x - a.bx[i], mp.L[0], mp.hL[0], mp.periodic[0]);
-  const double dy = min_image(y - a.by[i], mp.L[1], mp.hL[1], mp.periodic[1]);
-  const double dz = min_image(z - a.bz[i], mp.L[2], mp.hL[2], mp.periodic[2]);
+  const typename Mir::T m = encode_type<Mir>(mp, x, y, z, ti);
+  static_cast<typename Mir::T*>(a.M)[i] = m;
+  double dx, dy, dz;
+  if constexpr (Mir::kFixed) {
+    // displacement since the build from the 32-bit coordinates: the
+    // wrapping difference is the minimum image on a periodic axis
+    dx = static_cast<double>(Mir::wdiff(m.x, reinterpret_cast<const int32_t*>(a.bx)[i])) * mp.unit[0];
+    dy = static_cast<double>(Mir::wdiff(m.y, reinterpret_cast<const int32_t*>(a.by)[i])) * mp.unit[1];
+    dz = static_cast<double>(Mir::wdiff(m.z, reinterpret_cast<const int32_t*>(a.bz)[i])) * mp.unit[2];
+  } else {
+    dx = min_image(x - a.bx[i], mp.L[0], mp.hL[0], mp.periodic[0]);
+    dy = min_image(y - a.by[i], mp.L[1], mp.hL[1], mp.periodic[1]);
+    dz = min_image(z - a.bz[i], mp.L[2], mp.hL[2], mp.periodic[2]);
+  }
   if (dx * dx + dy * dy + dz * dz > a.skin_half_sq) a.flags[0] = 1;
   if (!(isfinite(x) && isfinite(y) && isfinite(z) && Mir::representable(mp, x, y, z)))
     a.flags[2] = 1;
@@ -666,9 +676,16 @@ __device__ __forceinline__ void copyback_atom(const StepArgs& a, const GridArgs&
   a.px[s] = x;
   a.py[s] = y;
   a.pz[s] = z;
-  a.bx[s] = x;
-  a.by[s] = y;
-  a.bz[s] = z;
+  if constexpr (Mir::kFixed) {
+    const MirrorParams& mp = mp_of<Mir>(a, s);
+    reinterpret_cast<int32_t*>(a.bx)[s] = Mir::enc1(x, mp.lo[0], mp.q[0]);
+    reinterpret_cast<int32_t*>(a.by)[s] = Mir::enc1(y, mp.lo[1], mp.q[1]);
+    reinterpret_cast<int32_t*>(a.bz)[s] = Mir::enc1(z, mp.lo[2], mp.q[2]);
+  } else {
+    a.bx[s] = x;
+    a.by[s] = y;
+    a.bz[s] = z;
+  }
   a.vx[s] = t.vx[s];
   a.vy[s] = t.vy[s];
   a.vz[s] = t.vz[s];
@@ -850,6 +867,153 @@ __global__ void __launch_bounds__(kBlock) k_rebuild_done(int32_t* flags, int32_t
 }
 
 // ---------------------------------------------------------------------------
+// Halos of the LDS-staged passes. Block b of those passes covers atoms
+// [1024 b, 1024 b + 1024); its halo is the set of atoms its rows name (the
+// block's own atoms included), in increasing index. A pass copies the halo's
+// records into LDS once and gathers neighbour records from there, so a
+// block reads ~3.5 records per atom from L2 instead of one per row slot.
+//
+// Built after every rebuild by one workgroup per block at a time -- as the
+// last phase of the fused rebuild, or by k_halo after a rebuild by separate
+// kernels (flags[4]): an LDS hash set collects the distinct indices of the block's rows,
+// a bitonic sort orders them, the list is written out, and every row slot
+// is rewritten as its position in the list (binary search). A halo larger
+// than stage_cap (or than kHaloMax) leaves the block on its 32-bit rows.
+// k_halo's last workgroup to finish clears flags[4].
+// ---------------------------------------------------------------------------
+constexpr int kHaloHashBits = 13;
+constexpr int kHaloHash = 1 << kHaloHashBits;  // hash slots
+constexpr int kHaloMax = 6144;                 // distinct atoms a halo may hold
+static_assert(kHaloMax <= 65536 && 4 * kHaloMax <= 3 * kHaloHash, "halo sizing");
+
+// Insert j into the open-addressing set `key` (empty = -1); multiplicative
+// (Fibonacci) hashing, linear probing. Counts new keys in *nd and raises
+// *ovf once more than kHaloMax are held; inserts stop then, so the table
+// never holds more than kHaloMax + (threads of the workgroup) keys and a
+// probe always meets its key or an empty slot.
+__device__ __forceinline__ void halo_insert(int32_t* key, int j, int* nd, volatile int* ovf) {
+  unsigned h = (static_cast<unsigned>(j) * 2654435761u) >> (32 - kHaloHashBits);
+  for (int p = 0; p < kHaloHash; ++p) {
+    const int old = atomicCAS(&key[h], -1, j);
+    if (old == j) return;
+    if (old == -1) {
+      if (atomicAdd(nd, 1) >= kHaloMax) *ovf = 1;
+      return;
+    }
+    h = (h + 1) & (kHaloHash - 1);
+  }
+  *ovf = 1;
+}
+
+constexpr int kHaloBlock = 1024;  // atoms per staged block (= the passes' workgroup)
+
+struct HaloLds {
+  int32_t key[kHaloHash];
+  int32_t srt[kHaloHash];
+  int nd, fill, ovf;
+};
+
+// Halos of blocks blockIdx.x, blockIdx.x + gridDim.x, ... (< nblocks), by the
+// whole workgroup (any size); its LDS is `L`.
+__device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, HaloLds& L) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int cap = a.cap;
+  int32_t* hj = const_cast<int32_t*>(a.hj);
+  int32_t* hn = const_cast<int32_t*>(a.hn);
+  int32_t* hmode = const_cast<int32_t*>(a.hmode);
+  uint16_t* nbr16 = const_cast<uint16_t*>(a.nbr16);
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    for (int t = tid; t < kHaloHash; t += nt) L.key[t] = -1;
+    if (tid == 0) {
+      L.nd = 0;
+      L.fill = 0;
+      L.ovf = 0;
+    }
+    __syncthreads();
+    volatile int* vovf = &L.ovf;
+    for (int t = tid; t < kHaloBlock && !*vovf; t += nt) {
+      const int i = b * kHaloBlock + t;
+      if (i >= a.natoms) break;
+      halo_insert(L.key, i, &L.nd, vovf);  // row padding names i itself
+      const int n = a.cnt[i];
+      for (int k = 0; k < n && !*vovf; ++k) halo_insert(L.key, a.nbr[ell(i, k, cap)], &L.nd, vovf);
+    }
+    __syncthreads();
+    const int H = L.nd;
+    if (L.ovf || H > a.stage_cap) {  // the same in every thread
+      if (tid == 0) {
+        hn[b] = H;
+        hmode[b] = 1;
+      }
+      __syncthreads();  // LDS is reused by the next block
+      continue;
+    }
+    for (int t = tid; t < kHaloHash; t += nt) {
+      const int v = L.key[t];
+      if (v != -1) L.srt[atomicAdd(&L.fill, 1)] = v;
+    }
+    int P = 1;
+    while (P < H) P <<= 1;
+    __syncthreads();
+    for (int t = H + tid; t < P; t += nt) L.srt[t] = 0x7fffffff;
+    __syncthreads();
+    // bitonic sort of srt[0, P), ascending
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int t = tid; t < (P >> 1); t += nt) {
+          const int lo = 2 * t - (t & (stride - 1));
+          const int hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const int x = L.srt[lo], y = L.srt[hi];
+          if ((x > y) == up) {
+            L.srt[lo] = y;
+            L.srt[hi] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int l = tid; l < H; l += nt) hj[static_cast<size_t>(b) * kHaloMax + l] = L.srt[l];
+    for (int t = tid; t < kHaloBlock; t += nt) {
+      const int i = b * kHaloBlock + t;
+      if (i >= a.natoms) break;
+      const int m = (a.cnt[i] + 7) & ~7;  // the padded row
+      for (int k = 0; k < m; ++k) {
+        const int j = a.nbr[ell(i, k, cap)];
+        int lo = 0, hi = H - 1;  // j is in srt[lo, hi]
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (L.srt[mid] < j)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        nbr16[ell16(i, k, cap)] = static_cast<uint16_t>(lo);
+      }
+    }
+    if (tid == 0) {
+      hn[b] = H;
+      hmode[b] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Standalone halo build after a rebuild by separate kernels (flags[4]).
+__global__ void __launch_bounds__(kHaloBlock) k_halo(StepArgs a, int nblocks, int32_t* done) {
+  if (!a.flags[4]) return;
+  __shared__ HaloLds L;
+  build_halos(a, nblocks, L);
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(done, 1) == static_cast<int>(gridDim.x) - 1) {
+      *done = 0;
+      a.flags[4] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Fused rebuild: the whole sort + list sequence in one launch, its phases
 // separated by a grid barrier, so a step that needs no rebuild costs one
 // early-returning launch instead of nine.
@@ -931,143 +1095,17 @@ k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double nea
     count = c > count ? c : count;
   }
   report_count<false>(a, count, nullptr);
+  if constexpr (Mir::kFixed) {
+    if (a.stage_cap > 0) {  // halos of the staged passes, from the new rows
+      __shared__ HaloLds L;
+      if (!grid_barrier(bar, a.flags)) return;
+      build_halos(a, (N + kHaloBlock - 1) / kHaloBlock, L);
+    }
+  }
   // every thread read the flag at entry, before the first barrier
   if (tid == 0) {
-    a.flags[4] = 1;  // the rows changed: halos are rebuilt next
     a.flags[0] = 0;
     *rebuilds += 1;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Halos of the LDS-staged passes. Block b of those passes covers atoms
-// [1024 b, 1024 b + 1024); its halo is the set of atoms its rows name (the
-// block's own atoms included), in increasing index. A pass copies the halo's
-// records into LDS once and gathers neighbour records from there, so a
-// block reads ~3.5 records per atom from L2 instead of one per row slot.
-//
-// Built after every rebuild (flags[4]) by one workgroup per block at a
-// time: an LDS hash set collects the distinct indices of the block's rows,
-// a bitonic sort orders them, the list is written out, and every row slot
-// is rewritten as its position in the list (binary search). A halo larger
-// than stage_cap (or than kHaloMax) leaves the block on its 32-bit rows.
-// The last workgroup to finish clears flags[4].
-// ---------------------------------------------------------------------------
-constexpr int kHaloHashBits = 13;
-constexpr int kHaloHash = 1 << kHaloHashBits;  // hash slots
-constexpr int kHaloMax = 6144;                 // distinct atoms a halo may hold
-static_assert(kHaloMax <= 65536 && 4 * kHaloMax <= 3 * kHaloHash, "halo sizing");
-
-// Insert j into the open-addressing set `key` (empty = -1); multiplicative
-// (Fibonacci) hashing, linear probing. Counts new keys in *nd and raises
-// *ovf once more than kHaloMax are held; inserts stop then, so the table
-// never holds more than kHaloMax + (threads of the workgroup) keys and a
-// probe always meets its key or an empty slot.
-__device__ __forceinline__ void halo_insert(int32_t* key, int j, int* nd, volatile int* ovf) {
-  unsigned h = (static_cast<unsigned>(j) * 2654435761u) >> (32 - kHaloHashBits);
-  for (int p = 0; p < kHaloHash; ++p) {
-    const int old = atomicCAS(&key[h], -1, j);
-    if (old == j) return;
-    if (old == -1) {
-      if (atomicAdd(nd, 1) >= kHaloMax) *ovf = 1;
-      return;
-    }
-    h = (h + 1) & (kHaloHash - 1);
-  }
-  *ovf = 1;
-}
-
-constexpr int kHaloBlock = 1024;  // atoms per staged block (= the passes' workgroup)
-
-__global__ void __launch_bounds__(kHaloBlock) k_halo(StepArgs a, int nblocks, int32_t* done) {
-  if (!a.flags[4]) return;
-  __shared__ int32_t key[kHaloHash];
-  __shared__ int32_t srt[kHaloHash];
-  __shared__ int nd, fill;
-  __shared__ int ovf;
-  const int tid = threadIdx.x;
-  const int cap = a.cap;
-  int32_t* hj = const_cast<int32_t*>(a.hj);
-  int32_t* hn = const_cast<int32_t*>(a.hn);
-  int32_t* hmode = const_cast<int32_t*>(a.hmode);
-  uint16_t* nbr16 = const_cast<uint16_t*>(a.nbr16);
-  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    for (int t = tid; t < kHaloHash; t += kHaloBlock) key[t] = -1;
-    if (tid == 0) {
-      nd = 0;
-      fill = 0;
-      ovf = 0;
-    }
-    __syncthreads();
-    const int i = b * kHaloBlock + tid;
-    const bool valid = i < a.natoms;
-    const int n = valid ? a.cnt[i] : 0;
-    volatile int* vovf = &ovf;
-    if (valid) halo_insert(key, i, &nd, vovf);  // row padding names i itself
-    for (int k = 0; k < n && !*vovf; ++k) halo_insert(key, a.nbr[ell(i, k, cap)], &nd, vovf);
-    __syncthreads();
-    const int H = nd;
-    if (ovf || H > a.stage_cap) {  // the same in every thread
-      if (tid == 0) {
-        hn[b] = H;
-        hmode[b] = 1;
-      }
-      __syncthreads();  // LDS is reused by the next block
-      continue;
-    }
-    for (int t = tid; t < kHaloHash; t += kHaloBlock) {
-      const int v = key[t];
-      if (v != -1) srt[atomicAdd(&fill, 1)] = v;
-    }
-    int P = 1;
-    while (P < H) P <<= 1;
-    __syncthreads();
-    for (int t = H + tid; t < P; t += kHaloBlock) srt[t] = 0x7fffffff;
-    __syncthreads();
-    // bitonic sort of srt[0, P), ascending
-    for (int size = 2; size <= P; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int t = tid; t < (P >> 1); t += kHaloBlock) {
-          const int lo = 2 * t - (t & (stride - 1));
-          const int hi = lo + stride;
-          const bool up = (lo & size) == 0;
-          const int x = srt[lo], y = srt[hi];
-          if ((x > y) == up) {
-            srt[lo] = y;
-            srt[hi] = x;
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int l = tid; l < H; l += kHaloBlock) hj[static_cast<size_t>(b) * kHaloMax + l] = srt[l];
-    if (valid) {
-      const int m = (n + 7) & ~7;  // the padded row
-      for (int k = 0; k < m; ++k) {
-        const int j = a.nbr[ell(i, k, cap)];
-        int lo = 0, hi = H - 1;  // j is in srt[lo, hi]
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (srt[mid] < j)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        nbr16[ell16(i, k, cap)] = static_cast<uint16_t>(lo);
-      }
-    }
-    if (tid == 0) {
-      hn[b] = H;
-      hmode[b] = 0;
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    __threadfence();
-    if (atomicAdd(done, 1) == static_cast<int>(gridDim.x) - 1) {
-      *done = 0;
-      a.flags[4] = 0;
-    }
   }
 }
 

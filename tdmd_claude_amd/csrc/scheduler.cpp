@@ -458,8 +458,7 @@ struct Engine final : EngineBase {
           S->a, S->g, S->t, S->r_list * S->r_list, S->cutoff * S->cutoff,
           const_cast<int32_t*>(S->a.nbr), const_cast<int32_t*>(S->a.cnt), S->d_bar,
           S->d_rebuilds);
-      ++S->launches;
-      halo(s);
+      ++S->launches;  // the fused kernel also builds the halos
       TDMD_LAUNCH_CHECK();
       return TDMD_OK;
     }
