@@ -74,18 +74,25 @@ def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[fl
 
 
 def bytes_per_atom(precision: str, mirror_fp64: bool, nbar: float, staged: float = 0.0,
-                   halo: float = 0.0) -> dict:
+                   halo: float = 0.0, nnear: float | None = None) -> dict:
     """Algorithmic HBM bytes per atom of the two steady-state passes: each
     per-atom array touched once per pass, the neighbour rows read once per
     pass, neighbour records served from cache (L2, or LDS for staged
-    blocks). Rows are 4 bytes a slot, or 2 bytes a slot plus the 4-byte halo
-    list (`halo` atoms per atom) for the fraction `staged` of blocks whose
-    halo is staged in LDS."""
-    rec = 32 if (precision == "fp64" or mirror_fp64) else 16  # M / M2 record
-    rows = staged * (2 * nbar + 4 * halo) + (1 - staged) * 4 * nbar
-    density = rec + 4 + rows + rec  # read M_i, count, rows; write M2_i
+    blocks). Rows are 4 bytes a slot; for the fraction `staged` of blocks
+    whose halo sits in LDS they are 2 bytes a slot plus the 4-byte halo list
+    (`halo` atoms per atom), and the force pass reads only the `nnear` slots
+    inside the cutoff at the build (its list of skin-shell slots inside the
+    cutoff, 0-3 entries, is not counted). Build positions for the skin test
+    are 32-bit coordinates with the fixed-point mirror, FP64 otherwise."""
+    fixed = precision == "mixed" and not mirror_fp64
+    rec = 16 if fixed else 32  # M / M2 record
+    xb = 12 if fixed else 24  # positions at the last build
+    nn = nbar if nnear is None else nnear
+    rows_d = staged * (2 * nbar + 4 * halo) + (1 - staged) * 4 * nbar
+    rows_f = staged * (2 * nn + 4 * halo) + (1 - staged) * 4 * nbar
+    density = rec + 4 + rows_d + rec  # read M_i, count, rows; write M2_i
     # force pass: read M2_i, count, type, x, v, x_build, rows; write x, v, M_i
-    force = rec + 4 + 4 + 24 + 24 + 24 + rows + 24 + 24 + rec
+    force = rec + 4 + 4 + 24 + 24 + xb + rows_f + 24 + 24 + rec
     return dict(density=density, force=force, step=density + force)
 
 
@@ -342,13 +349,14 @@ def main() -> int:
     nve = FastPipelineScheduler.for_system(
         state, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
     nve.run_until(0)
-    nbar = nve.thermo()["mean_nbr"]
+    obs = nve.observables()
+    nbar, nnear = obs["mean_neighbors"], obs["mean_near_neighbors"]
     t_dens, t_force = nve.time_passes(30)
     halo = nve.halo_info()
     nve.close()
     staged = halo["staged"] / halo["blocks"] if halo["blocks"] else 0.0
     halo_pa = halo["halo_atoms"] / N if halo["blocks"] else 0.0
-    bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar, staged, halo_pa)
+    bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar, staged, halo_pa, nnear)
     traffic = measured_traffic(N, args.mirror_fp64)
     achieved = N * bpa["force"] / (t_force * 1e-3) / 1e9
     log(f"passes: density {t_dens:.4f} ms, force {t_force:.4f} ms; force pass "
@@ -466,6 +474,7 @@ def main() -> int:
             "density_pass_ms": round(t_dens, 5),
             "halo_staged_blocks": f"{halo['staged']}/{halo['blocks']}",
             "halo_atoms_per_atom": round(halo_pa, 3),
+            "near_slots_per_atom": round(nnear, 2),
         },
         "cpu_baseline": cpu,
         "extras": extras,

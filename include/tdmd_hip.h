@@ -371,7 +371,8 @@ typedef struct tdmd_observables {
   double virial, volume, pressure;
   int32_t max_neighbors;
   int32_t pad;
-  double mean_neighbors;
+  double mean_neighbors;      /* mean row length (pairs within r_list)     */
+  double mean_near_neighbors; /* mean pairs within the cutoff at the build */
 } tdmd_observables;
 int tdmd_sched_observables(tdmd_sched* s, tdmd_observables* out);
 /* Timing window (no reference counterpart; measurement aid): runs `steps`

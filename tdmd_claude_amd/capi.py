@@ -107,6 +107,7 @@ class Observables(C.Structure):
         ("max_neighbors", C.c_int32),
         ("pad", C.c_int32),
         ("mean_neighbors", C.c_double),
+        ("mean_near_neighbors", C.c_double),
     ]
 
 

@@ -1402,6 +1402,11 @@ int tdmd_sched_observables(tdmd_sched* S, tdmd_observables* o) {
   o->pressure = (2.0 * th[1] + sc[3]) / (3.0 * vol) * 1.602176634e6;
   o->max_neighbors = static_cast<int32_t>(th[4]);
   o->mean_neighbors = th[5];
+  std::vector<int32_t> nn(static_cast<size_t>(S->natoms));
+  TDMD_HIP_TRY(hipMemcpy(nn.data(), S->a.cntn, nn.size() * 4, hipMemcpyDeviceToHost));
+  int64_t sn = 0;
+  for (int32_t v : nn) sn += v;
+  o->mean_near_neighbors = static_cast<double>(sn) / S->natoms;
   return TDMD_OK;
 }
 
