@@ -54,11 +54,13 @@ __device__ __forceinline__ void emit_partial(double v, double* partials, int slo
 }
 
 // Kick-and-drift for atom i with a given kick factor and an optional pre
-// scale of v (NHC). Writes x, v and the M record; sets the rebuild flag when
-// the atom has moved more than skin/2 since the last build (minimum image).
+// scale of v (NHC). Writes x, v and the gather record (into `rec`: M, or the
+// Morse pass's other record buffer); sets the rebuild flag when the atom has
+// moved more than skin/2 since the last build (minimum image).
 template <class R, class Mir>
 __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vscale,
-                                           double kick_dt, double f0, double f1, double f2) {
+                                           double kick_dt, double f0, double f1, double f2,
+                                           void* rec) {
   const int ti = a.types[i] - 1;
   const double mass = a.masses[ti + 1];
   const double factor = kick_dt / (mass * kMvv2e);
@@ -76,7 +78,7 @@ __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vsca
   a.py[i] = y;
   a.pz[i] = z;
   const typename Mir::T m = encode_type<Mir>(mp, x, y, z, ti);
-  static_cast<typename Mir::T*>(a.M)[i] = m;
+  static_cast<typename Mir::T*>(rec)[i] = m;
   double dx, dy, dz;
   if constexpr (Mir::kFixed) {
     // displacement since the build from the 32-bit coordinates: the
@@ -94,14 +96,16 @@ __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vsca
     a.flags[2] = 1;
 }
 
-// Force-pass epilogue.
+// Force-pass epilogue. The full kick-drift writes the next step's records
+// into `rec` (M for the EAM passes, which gather M2; the other buffer of the
+// pair for the Morse pass, which gathers M).
 template <class R, class Mir, int kEpi>
 __device__ __forceinline__ double epilogue(const StepArgs& a, int i, double fx, double fy,
-                                           double fz) {
+                                           double fz, void* rec = nullptr) {
   const R f0 = static_cast<R>(fx), f1 = static_cast<R>(fy), f2 = static_cast<R>(fz);
   if (kEpi == kFullKickDrift) {
     kick_drift<R, Mir>(a, i, 1.0, a.dt, static_cast<double>(f0), static_cast<double>(f1),
-                       static_cast<double>(f2));
+                       static_cast<double>(f2), rec ? rec : a.M);
     return 0.0;
   }
   rp<R>(a.fx)[i] = f0;
@@ -136,7 +140,8 @@ __global__ void __launch_bounds__(kBlock) k_kick_drift(StepArgs a) {
   if (i >= a.natoms) return;
   const double s = kNvt ? a.nhc[26] : 1.0;
   kick_drift<R, Mir>(a, i, s, kFull ? a.dt : 0.5 * a.dt, static_cast<double>(rp<R>(a.fx)[i]),
-                     static_cast<double>(rp<R>(a.fy)[i]), static_cast<double>(rp<R>(a.fz)[i]));
+                     static_cast<double>(rp<R>(a.fy)[i]), static_cast<double>(rp<R>(a.fz)[i]),
+                     a.M);
 }
 
 // Mirror refresh (after upload or restore): M[i] = {x, type}.
@@ -471,8 +476,8 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
 
 template <class R, class Mir, int kEpi, bool kEnergy>
 __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p) {
-  // The Morse pass gathers from M, which the kick-drift epilogue rewrites.
-  static_assert(kEpi != kFullKickDrift, "Morse kick-drift runs as its own launch");
+  // The pass gathers from M; with the fused kick-drift the next step's
+  // records go to M2 and the host swaps the two buffers after the launch.
   using T = typename Mir::T;
   const int blk = xcd_block(a.nblocks);
   const int i = blk * kBlock + threadIdx.x;
@@ -499,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
         fz = fma(fpair, dz, fz);
       }
     });
-    const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
+    const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz, a.M2);
     red = kEnergy ? pe : ke;
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);

@@ -42,13 +42,13 @@ using namespace sched;
 // Operation codes of a step sequence (also the graph-cache key alphabet).
 enum Op : char {
   OP_BOOT = 'b',        // half kick + drift (window start, NVE)
-  OP_KD_FULL = 'k',     // full kick + drift (Morse NVE, steps 2..n)
+  OP_KD_FULL = 'k',     // full kick + drift as its own launch (Morse pass timing)
   OP_NVT_PRE = 'p',     // chain half-step before the step
   OP_KD_NVT = 'q',      // scaled half kick + drift
   OP_REB_FORCED = 'R',  // set flag + rebuild
   OP_REB_COND = 'r',    // rebuild if the device flag is set
   OP_DENS = 'd',        // EAM density pass
-  OP_F_FULL = 'F',      // force + next step's kick-drift (EAM)
+  OP_F_FULL = 'F',      // force + next step's kick-drift (Morse: into the other record buffer)
   OP_F_ONLY = 'o',      // force, store F
   OP_F_HALF = 'h',      // force + closing half kick
   OP_F_KICK2 = '2',     // force + half kick + KE partials (NVT)
@@ -118,11 +118,16 @@ struct tdmd_sched {
   std::vector<int32_t> h_types, h_ids;
   std::vector<double> h_masses;
   bool uploaded = false, initial_forces = false;
+  // Morse: the gather records alternate between two buffers (the fused
+  // kick-drift writes the next step's into the one not being gathered);
+  // true when a.M / a.M2 are exchanged relative to their allocation
+  bool rec_swapped = false;
   int current_step = 0, steps_since = 0;
   int64_t ticks = 0, launches = 0, host_rebuilds = 0, graph_launches = 0;
   std::map<std::string, hipGraphExec_t> graphs;
   std::map<std::string, hipGraph_t> graph_defs;
   std::map<std::string, int64_t> graph_kernels;
+  std::map<std::string, bool> graph_swaps;  // the graph exchanges the record buffers
   // batches of independent boxes (replicas), atoms of replica r in slots
   // [rep_off[r], rep_off[r+1])
   bool multibox = false;
@@ -446,8 +451,12 @@ struct Engine final : EngineBase {
         go(k_eam_force<R, Mir, kEpi, kEnergy, false>, nb, kBlock, 0, s, S->a, meta(), coeff(),
            rc_sq());
     } else {
-      if constexpr (kEpi != kFullKickDrift)
-        go(k_morse_force<R, Mir, kEpi, kEnergy>, nb, kBlock, 0, s, S->a, mp());
+      go(k_morse_force<R, Mir, kEpi, kEnergy>, nb, kBlock, 0, s, S->a, mp());
+      if constexpr (kEpi == kFullKickDrift) {
+        // the epilogue wrote the next step's records into M2: it is now M
+        std::swap(S->a.M, S->a.M2);
+        S->rec_swapped = !S->rec_swapped;
+      }
     }
   }
 
@@ -883,6 +892,7 @@ int first_build(tdmd_sched* S) {
     S->graphs.clear();
     S->graph_defs.clear();
     S->graph_kernels.clear();
+    S->graph_swaps.clear();
   }
   rc = S->eng->fill_pass(s);
   if (rc) return rc;
@@ -901,9 +911,12 @@ int run_ops(tdmd_sched* S, const std::string& ops) {
     }
     return TDMD_OK;
   }
-  auto it = S->graphs.find(ops);
+  // a graph holds the record buffers of the state it was captured in
+  const std::string key = ops + (S->rec_swapped ? "/1" : "/0");
+  auto it = S->graphs.find(key);
   if (it == S->graphs.end()) {
     const int64_t saved = S->launches;
+    const bool swapped0 = S->rec_swapped;
     TDMD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (char op : ops) {
       const int rc = S->eng->launch(op, s);
@@ -917,12 +930,22 @@ int run_ops(tdmd_sched* S, const std::string& ops) {
     TDMD_HIP_TRY(hipStreamEndCapture(s, &gdef));
     hipGraphExec_t exec;
     TDMD_HIP_TRY(hipGraphInstantiate(&exec, gdef, nullptr, nullptr, 0));
-    S->graph_defs[ops] = gdef;
-    S->graph_kernels[ops] = S->launches - saved;
-    it = S->graphs.emplace(ops, exec).first;
+    S->graph_defs[key] = gdef;
+    S->graph_kernels[key] = S->launches - saved;
+    S->graph_swaps[key] = S->rec_swapped != swapped0;
+    // capture enqueued nothing: back to the state before it, then replay
+    if (S->graph_swaps[key]) {
+      std::swap(S->a.M, S->a.M2);
+      S->rec_swapped = swapped0;
+    }
+    it = S->graphs.emplace(key, exec).first;
     S->launches = saved;
   }
-  S->launches += S->graph_kernels[ops];  // kernels the graph replays
+  S->launches += S->graph_kernels[key];  // kernels the graph replays
+  if (S->graph_swaps[key]) {  // the replay leaves the records in the other buffer
+    std::swap(S->a.M, S->a.M2);
+    S->rec_swapped = !S->rec_swapped;
+  }
   TDMD_HIP_TRY(hipGraphLaunch(it->second, s));
   ++S->graph_launches;
   return TDMD_OK;
@@ -966,9 +989,7 @@ int run_window(tdmd_sched* S, int n) {
       st += static_cast<char>(OP_KD_NVT);
     } else if (k == 1) {
       st += static_cast<char>(OP_BOOT);
-    } else if (!eam) {
-      st += static_cast<char>(OP_KD_FULL);
-    }
+    }  // NVE: later steps' kick-drift is the previous force pass's epilogue
     if (disp) {
       st += static_cast<char>(OP_REB_COND);
     } else if (S->steps_since >= S->cfg.rebuild_every) {
@@ -983,7 +1004,7 @@ int run_window(tdmd_sched* S, int n) {
     } else if (k == n) {
       st += static_cast<char>(OP_F_HALF);
     } else {
-      st += static_cast<char>(eam ? OP_F_FULL : OP_F_ONLY);
+      st += static_cast<char>(OP_F_FULL);
     }
     if (nvt && k == n) st += static_cast<char>(OP_NVT_FINAL);
     steps.push_back(st);
@@ -1156,6 +1177,7 @@ int tdmd_sched_autotune(tdmd_sched* S, int32_t iters, int32_t out[2]) {
   S->graphs.clear();
   S->graph_defs.clear();
   S->graph_kernels.clear();
+  S->graph_swaps.clear();
   S->cfg.pair_unroll = out[0];
   S->cfg.lds_budget = out[1];
   return check_flags(S);

@@ -112,13 +112,16 @@ def test_thermo_matches_oracle():
         assert th["temp"] == pytest.approx(rth[0, 4], rel=1e-12)
 
 
-def test_graph_and_direct_launch_are_bitwise_identical():
+@pytest.mark.parametrize("pot", ["johnson", "morse"])
+def test_graph_and_direct_launch_are_bitwise_identical(pot):
+    # Morse alternates its record buffers every step: graphs are keyed by the
+    # buffer state they were captured in (windows of odd length here)
     s = fcc(5)
-    say("graph vs direct")
+    say("graph vs direct", pot)
     out = []
     for g in (True, False):
-        _, pos, vel, f = gpu_run(s, "johnson", 40, precision="mixed", use_graph=g,
-                                 rebuild_policy="displacement")
+        _, pos, vel, f = gpu_run(s, pot, 41, chunks=[7, 24, 41], precision="mixed",
+                                 use_graph=g, rebuild_policy="displacement")
         out.append((pos, vel, f))
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
@@ -138,11 +141,12 @@ def test_download_returns_upload_order():
     assert np.abs(vel_b - vel_a[perm]).max() < 1e-9
 
 
-def test_chunked_run_until_reproduces_single_window():
+@pytest.mark.parametrize("pot", ["johnson", "morse"])
+def test_chunked_run_until_reproduces_single_window(pot):
     s = fcc(5)
-    say("chunked run_until")
-    _, pos_a, vel_a, _ = gpu_run(s, "johnson", 60, precision="fp64")
-    _, pos_b, vel_b, _ = gpu_run(s, "johnson", 60, chunks=[10, 35, 60], precision="fp64")
+    say("chunked run_until", pot)
+    _, pos_a, vel_a, _ = gpu_run(s, pot, 60, precision="fp64")
+    _, pos_b, vel_b, _ = gpu_run(s, pot, 60, chunks=[10, 35, 60], precision="fp64")
     assert np.abs(pos_a - pos_b).max() < 1e-11
     assert np.abs(vel_a - vel_b).max() < 1e-9
 
@@ -224,6 +228,16 @@ def test_launch_accounting():
     per_step = (b.kernel_launches - a.kernel_launches) / 100.0
     assert 2.0 <= per_step <= 3.2
     assert b.ticks - a.ticks == 100
+    # Morse: force(+kick-drift) and the rebuild check per step
+    sched = FastPipelineScheduler.for_system(s, MORSE_CU, FastPipelineConfig(
+        precision="mixed", rebuild_policy="displacement"))
+    sched.run_until(1)
+    a = sched.stats()
+    sched.run_until(101)
+    b = sched.stats()
+    per_step = (b.kernel_launches - a.kernel_launches) / 100.0
+    say("  morse launches per step", per_step)
+    assert 2.0 <= per_step <= 2.2
 
 
 def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged():
