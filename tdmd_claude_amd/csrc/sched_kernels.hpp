@@ -195,6 +195,13 @@ enum Epilogue : int {
   kKick2Ke = 3       // NVT: v += dt/2 F/m; store F, v; KE partials
 };
 
+// Nose-Hoover chain constants (device_nose_hoover.cu:206-289).
+struct NhcParams {
+  int M;
+  int n_dof;
+  double t_target, t_period, dt;
+};
+
 struct StepArgs {
   int natoms;
   int nblocks;  // grid size (multiple of 8)

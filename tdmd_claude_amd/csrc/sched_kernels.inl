@@ -180,12 +180,6 @@ __global__ void k_scale_v(StepArgs a) {
 //   [27] reduced sum scratch
 // ---------------------------------------------------------------------------
 
-struct NhcParams {
-  int M;
-  int n_dof;
-  double t_target, t_period, dt;
-};
-
 // Before the VV step: half-step on the KE of the actual velocities
 // (vscale^2 * stored KE), kick scale = s1 * vscale.
 __global__ void k_nhc_pre(double* nhc, NhcParams p) {
@@ -197,49 +191,50 @@ __global__ void k_nhc_pre(double* nhc, NhcParams p) {
   nhc[24] = 1.0;
 }
 
-// After the VV step: KE from the per-block partials of the kick-2 epilogue,
-// half-step, pending scale = s2. kPre: the next step's k_nhc_pre follows in
-// the same thread (same operations, one launch less per step).
+// Sum of n per-block partials by one wavefront, in a fixed order: lane l adds
+// k = l, l + 64, ... in turn, then a butterfly combines the lane sums. IEEE
+// addition commutes, so every lane ends with the same bits, and every caller
+// summing the same partials gets the same result.
+__device__ __forceinline__ double wave_sum_partials(const double* partials, int n) {
+  double t = 0.0;
+  for (int k = static_cast<int>(threadIdx.x & 63); k < n; k += 64) t += partials[k];
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) t += __shfl_xor(t, w, 64);
+  return t;
+}
+
+// After the VV step (one thread): half-step on the KE of the kick-2
+// velocities, pending scale = s2. With `pre`, also the next step's
+// half-step on the scaled KE: kick scale = s1 * s2.
+__device__ __forceinline__ void nhc_after_step(double* nhc, double ke, const NhcParams& p,
+                                               bool pre) {
+  const double s2 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
+                                  p.n_dof, ke);
+  nhc[25] = ke;
+  nhc[24] = s2;
+  if (pre) {
+    const double ke1 = s2 * s2 * ke;
+    const double s1 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
+                                    p.n_dof, ke1);
+    nhc[26] = s1 * s2;
+    nhc[24] = 1.0;
+  }
+}
+
+// Chain update after the kick-2 force pass (one wavefront).
 template <bool kPre>
 __global__ void k_nhc_post(double* nhc, const double* partials, int n, NhcParams p) {
-  __shared__ double s[256];
-  double t = 0.0;
-  for (int k = threadIdx.x; k < n; k += 256) t += partials[k];
-  s[threadIdx.x] = t;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double ke = s[0];
-    const double s2 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
-                                    p.n_dof, ke);
-    nhc[25] = ke;
-    nhc[24] = s2;
-    if (kPre) {
-      const double ke1 = nhc[24] * nhc[24] * nhc[25];
-      const double s1 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
-                                      p.n_dof, ke1);
-      nhc[26] = s1 * nhc[24];
-      nhc[24] = 1.0;
-    }
-  }
+  if (threadIdx.x >= 64) return;
+  const double ke = wave_sum_partials(partials, n);
+  if (threadIdx.x == 0) nhc_after_step(nhc, ke, p, kPre);
 }
 
 // Stored-KE refresh (after upload, or after the pending scale was applied).
 __global__ void k_nhc_set_ke(double* nhc, const double* partials, int n, int commit_scale) {
-  __shared__ double s[256];
-  double t = 0.0;
-  for (int k = threadIdx.x; k < n; k += 256) t += partials[k];
-  s[threadIdx.x] = t;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
+  if (threadIdx.x >= 64) return;
+  const double ke = wave_sum_partials(partials, n);
   if (threadIdx.x == 0) {
-    nhc[25] = s[0];
+    nhc[25] = ke;
     if (commit_scale) nhc[24] = 1.0;
   }
 }
@@ -522,6 +517,8 @@ struct GridArgs {
   double inv_cs[3];
   int ncells;
   const int32_t* rank_of;  // linear cell id -> Morton rank
+  const int32_t* around;   // [ncells][27] by rank: ranks of the 27 cells
+                           // around the cell (periodic wrap), increasing
   int32_t* ccount;         // [ncells] by rank
   int32_t* cstart;         // [ncells] by rank
   int32_t* cursor;         // [ncells] by rank
@@ -605,34 +602,29 @@ __device__ __forceinline__ void bin_atom(const StepArgs& a, const GridArgs& g, i
 }
 
 // Exclusive scan of the cell counts by one workgroup of BLOCK threads
-// (atomics-free).
+// (atomics-free): thread t owns a contiguous chunk of cells, sums it, the
+// workgroup scans the chunk sums, and each thread writes its chunk's starts.
 template <int BLOCK>
 __device__ __forceinline__ void scan_cells(const GridArgs& g) {
   __shared__ int wsum[BLOCK / 64];
-  __shared__ int carry_s;
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int start = 0; start < g.ncells; start += BLOCK) {
-    const int c = start + static_cast<int>(threadIdx.x);
-    const int v = c < g.ncells ? g.ccount[c] : 0;
-    int incl = v;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int o = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += o;
-    }
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int base = carry_s;
-    for (int k = 0; k < w; ++k) base += wsum[k];
-    if (c < g.ncells) g.cstart[c] = base + incl - v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int t = 0;
-      for (int k = 0; k < BLOCK / 64; ++k) t += wsum[k];
-      carry_s += t;
-    }
-    __syncthreads();
+  const int chunk = (g.ncells + BLOCK - 1) / BLOCK;
+  const int c0 = min(static_cast<int>(threadIdx.x) * chunk, g.ncells);
+  const int c1 = min(c0 + chunk, g.ncells);
+  int v = 0;
+  for (int c = c0; c < c1; ++c) v += g.ccount[c];
+  int incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int base = incl - v;
+  for (int k = 0; k < w; ++k) base += wsum[k];
+  for (int c = c0; c < c1; ++c) {
+    g.cstart[c] = base;
+    base += g.ccount[c];
   }
 }
 
@@ -729,40 +721,36 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
   // cell rank, so each row comes out in increasing atom index, and the
   // lanes of a wave (neighbouring atoms with overlapping neighbourhoods)
   // reach the same stretch of the record array at the same slot.
-  int rk[27];
-#pragma unroll
+  const int32_t* __restrict__ around =
+      g.around + 27 * static_cast<size_t>(
+                          g.rank_of[cg.off + (c[2] * cg.n[1] + c[1]) * cg.n[0] + c[0]]);
   for (int t = 0; t < 27; ++t) {
-    int nx = c[0] + t % 3 - 1, ny = c[1] + (t / 3) % 3 - 1, nz = c[2] + t / 9 - 1;
-    nx = nx < 0 ? nx + cg.n[0] : (nx >= cg.n[0] ? nx - cg.n[0] : nx);
-    ny = ny < 0 ? ny + cg.n[1] : (ny >= cg.n[1] ? ny - cg.n[1] : ny);
-    nz = nz < 0 ? nz + cg.n[2] : (nz >= cg.n[2] ? nz - cg.n[2] : nz);
-    rk[t] = g.rank_of[cg.off + (nz * cg.n[1] + ny) * cg.n[0] + nx];
-  }
-  unsigned left = (1u << 27) - 1u;
-  for (int step = 0; step < 27; ++step) {
-    int r = 0x7fffffff, bt = 0;
-#pragma unroll
-    for (int t = 0; t < 27; ++t) {
-      const bool take = ((left >> t) & 1u) != 0u && rk[t] < r;
-      r = take ? rk[t] : r;
-      bt = take ? t : bt;
-    }
-    left &= ~(1u << bt);
+    const int r = around[t];
     const int start = g.cstart[r], end = start + g.ccount[r];
-    for (int j = start; j < end; ++j) {
-      if (j == i) continue;
-      const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
-      const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
-      const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
-      const double r2 = r2_unfused(ddx, ddy, ddz);
-      if (r2 < r_list_sq) {
-        if (!kCount && nn + nf < cap) {
-          if (r2 < near_sq)
-            nbr[ell(i, nn++, cap)] = j;
-          else
-            nbr[ell(i, cap - 1 - nf++, cap)] = j;
+    // candidates in groups of four: the group's coordinates are loaded
+    // together, then tested and appended in index order
+    for (int j0 = start; j0 < end; j0 += 4) {
+      double r2q[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = min(j0 + q, end - 1);
+        const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
+        const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
+        const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
+        r2q[q] = r2_unfused(ddx, ddy, ddz);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        if (j < end && j != i && r2q[q] < r_list_sq) {
+          if (!kCount && nn + nf < cap) {
+            if (r2q[q] < near_sq)
+              nbr[ell(i, nn++, cap)] = j;
+            else
+              nbr[ell(i, cap - 1 - nf++, cap)] = j;
+          }
+          ++count;
         }
-        ++count;
       }
     }
   }
@@ -891,13 +879,18 @@ constexpr int kHaloHash = 1 << kHaloHashBits;  // hash slots
 constexpr int kHaloMax = 6144;                 // distinct atoms a halo may hold
 static_assert(kHaloMax <= 65536 && 4 * kHaloMax <= 3 * kHaloHash, "halo sizing");
 
-// Insert j into the open-addressing set `key` (empty = -1); multiplicative
-// (Fibonacci) hashing, linear probing. Counts new keys in *nd and raises
+// Multiplicative (Fibonacci) hash of an atom index onto the set's slots.
+__device__ __forceinline__ unsigned halo_hash(int j) {
+  return (static_cast<unsigned>(j) * 2654435761u) >> (32 - kHaloHashBits);
+}
+
+// Insert j into the open-addressing set `key` (empty = -1), linear
+// probing. Counts new keys in *nd and raises
 // *ovf once more than kHaloMax are held; inserts stop then, so the table
 // never holds more than kHaloMax + (threads of the workgroup) keys and a
 // probe always meets its key or an empty slot.
 __device__ __forceinline__ void halo_insert(int32_t* key, int j, int* nd, volatile int* ovf) {
-  unsigned h = (static_cast<unsigned>(j) * 2654435761u) >> (32 - kHaloHashBits);
+  unsigned h = halo_hash(j);
   for (int p = 0; p < kHaloHash; ++p) {
     const int old = atomicCAS(&key[h], -1, j);
     if (old == j) return;
@@ -918,10 +911,19 @@ struct HaloLds {
   int nd, fill, ovf;
 };
 
-// Halos of blocks blockIdx.x, blockIdx.x + gridDim.x, ... (< nblocks), by the
-// whole workgroup (any size); its LDS is `L`.
+// Hash slot of j, which the set holds.
+__device__ __forceinline__ int halo_find(const int32_t* key, int j) {
+  unsigned h = halo_hash(j);
+  while (key[h] != j) h = (h + 1) & (kHaloHash - 1);
+  return static_cast<int>(h);
+}
+
+// Halos of blocks blockIdx.x, blockIdx.x + gridDim.x, ... (< nblocks), by a
+// workgroup of kHaloBlock threads; its LDS is `L`.
 __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, HaloLds& L) {
-  const int tid = threadIdx.x, nt = blockDim.x;
+  constexpr int nt = kHaloBlock;
+  constexpr int kPer = (kHaloMax + nt - 1) / nt;  // sorted keys per thread
+  const int tid = threadIdx.x;
   const int cap = a.cap;
   int32_t* hj = const_cast<int32_t*>(a.hj);
   int32_t* hn = const_cast<int32_t*>(a.hn);
@@ -978,22 +980,29 @@ __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, Halo
         __syncthreads();
       }
     }
-    for (int l = tid; l < H; l += nt) hj[static_cast<size_t>(b) * kHaloMax + l] = L.srt[l];
-    for (int t = tid; t < kHaloBlock; t += nt) {
-      const int i = b * kHaloBlock + t;
-      if (i >= a.natoms) break;
-      const int m = (a.cnt[i] + 7) & ~7;  // the padded row
-      for (int k = 0; k < m; ++k) {
-        const int j = a.nbr[ell(i, k, cap)];
-        int lo = 0, hi = H - 1;  // j is in srt[lo, hi]
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (L.srt[mid] < j)
-            lo = mid + 1;
-          else
-            hi = mid;
+    // The list goes out; then srt is reused to hold each key's list
+    // position at the key's hash slot (every thread first takes its keys
+    // into registers), so a row slot's position is one hash probe away.
+    int v[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int l = q * nt + tid;
+      v[q] = l < H ? L.srt[l] : -1;
+      if (l < H) hj[static_cast<size_t>(b) * kHaloMax + l] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q)
+      if (v[q] >= 0) L.srt[halo_find(L.key, v[q])] = q * nt + tid;
+    __syncthreads();
+    {
+      const int i = b * kHaloBlock + tid;
+      if (i < a.natoms) {
+        const int m = (a.cnt[i] + 7) & ~7;  // the padded row
+        for (int k = 0; k < m; ++k) {
+          const int j = a.nbr[ell(i, k, cap)];
+          nbr16[ell16(i, k, cap)] = static_cast<uint16_t>(L.srt[halo_find(L.key, j)]);
         }
-        nbr16[ell16(i, k, cap)] = static_cast<uint16_t>(lo);
       }
     }
     if (tid == 0) {
@@ -1028,8 +1037,16 @@ __global__ void __launch_bounds__(kHaloBlock) k_halo(StepArgs a, int nblocks, in
 // the barrier can complete. The barrier is a count + generation pair in
 // device memory; its wait is bounded: a wait longer than kBarrierSpins
 // polls sets flags[3] and the kernel returns (the host reports the error)
-// instead of hanging. Writes of one phase are published to the other XCDs'
-// caches by agent-scope fences on both sides of the barrier.
+// instead of hanging.
+//
+// Only thread 0 of each workgroup touches the barrier. Its arrival is a
+// release and its wait ends with an acquire fence at agent scope, and the
+// workgroup barriers on either side extend both to the other threads: one
+// L2 write-back and one invalidate per workgroup and phase, not one per
+// wave. The polls themselves are relaxed loads: an acquiring poll would
+// invalidate the XCD's L2 at every poll, under the workgroups still working
+// (phases such as the cell scan run on one workgroup), and polls back off
+// with a growing sleep.
 // ---------------------------------------------------------------------------
 
 struct GridBarrier {
@@ -1041,12 +1058,11 @@ constexpr long kBarrierSpins = 1L << 22;
 
 __device__ __forceinline__ bool grid_barrier(GridBarrier* b, int32_t* flags) {
   __shared__ int ok;
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     ok = 1;
     const unsigned int gen =
-        __hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int arrived =
         __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == gridDim.x - 1) {
@@ -1054,28 +1070,35 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, int32_t* flags) {
       __hip_atomic_store(&b->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       long spins = 0;
-      while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(2);
+      while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (spins < 16)
+          __builtin_amdgcn_s_sleep(2);
+        else if (spins < 256)
+          __builtin_amdgcn_s_sleep(16);
+        else
+          __builtin_amdgcn_s_sleep(64);
         if (++spins > kBarrierSpins) {
           ok = 0;
           atomicExch(&flags[3], 1);
           break;
         }
       }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
   }
   __syncthreads();
-  __threadfence();
   return ok != 0;
 }
 
+constexpr int kRebuildBlock = 1024;  // workgroup of the fused rebuild
+
 template <class R, class Mir>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRebuildBlock)
 k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double near_sq,
                 int32_t* nbr, int32_t* cnt, GridBarrier* bar, int64_t* rebuilds) {
   if (!a.flags[0]) return;  // uniform: nothing writes the flag before the last phase
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
-  const int nthr = gridDim.x * kBlock;
+  const int tid = blockIdx.x * kRebuildBlock + threadIdx.x;
+  const int nthr = gridDim.x * kRebuildBlock;
   const int N = a.natoms;
   for (int r = tid; r < g.ncells; r += nthr) {
     g.ccount[r] = 0;
@@ -1084,11 +1107,17 @@ k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double nea
   if (!grid_barrier(bar, a.flags)) return;
   for (int i = tid; i < N; i += nthr) bin_atom<Mir>(a, g, i);
   if (!grid_barrier(bar, a.flags)) return;
-  if (blockIdx.x == 0) scan_cells<kBlock>(g);
+  if (blockIdx.x == 0) scan_cells<kRebuildBlock>(g);
   if (!grid_barrier(bar, a.flags)) return;
   for (int i = tid; i < N; i += nthr) scatter_atom(g, i);
   if (!grid_barrier(bar, a.flags)) return;
-  for (int r = tid; r < g.ncells; r += nthr) order_cell(g, r);
+  {
+    // one cell per thread, cells dealt to the workgroups wave by wave so the
+    // few cells (far fewer than threads) spread over every CU
+    const int nw = gridDim.x * (kRebuildBlock / 64);
+    const int wv = (threadIdx.x >> 6) * gridDim.x + blockIdx.x;
+    for (int r = wv * 64 + (threadIdx.x & 63); r < g.ncells; r += nw * 64) order_cell(g, r);
+  }
   if (!grid_barrier(bar, a.flags)) return;
   for (int s = tid; s < N; s += nthr) permute_atom<R>(a, g, t, s);
   if (!grid_barrier(bar, a.flags)) return;
@@ -1104,6 +1133,7 @@ k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double nea
     if (a.stage_cap > 0) {  // halos of the staged passes, from the new rows
       __shared__ HaloLds L;
       if (!grid_barrier(bar, a.flags)) return;
+      static_assert(kRebuildBlock == kHaloBlock, "halos are built by whole workgroups");
       build_halos(a, (N + kHaloBlock - 1) / kHaloBlock, L);
     }
   }

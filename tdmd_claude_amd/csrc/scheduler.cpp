@@ -250,9 +250,9 @@ struct Engine final : EngineBase {
       TDMD_HIP_TRY(hipGetDevice(&dev));
       TDMD_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
       TDMD_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(&k_rebuild_fused<R, Mir>), kBlock, 0));
+          &per_cu, reinterpret_cast<const void*>(&k_rebuild_fused<R, Mir>), kRebuildBlock, 0));
       // at most two workgroups per CU: a margin under the occupancy limit
-      const int need = std::max(grid_for(S->natoms), 1);
+      const int need = std::max((S->natoms + kRebuildBlock - 1) / kRebuildBlock, 1);
       S->nb_rebuild = std::max(1, std::min(std::min(per_cu, 2) * ncu, need));
       if (per_cu < 1) S->nb_rebuild = 0;  // cannot guarantee residency: separate kernels
     }
@@ -463,7 +463,7 @@ struct Engine final : EngineBase {
   // Conditional rebuild: one fused launch, or the separate kernels.
   int rebuild(hipStream_t s) {
     if (S->nb_rebuild > 0) {
-      k_rebuild_fused<R, Mir><<<S->nb_rebuild, kBlock, 0, s>>>(
+      k_rebuild_fused<R, Mir><<<S->nb_rebuild, kRebuildBlock, 0, s>>>(
           S->a, S->g, S->t, S->r_list * S->r_list, S->cutoff * S->cutoff,
           const_cast<int32_t*>(S->a.nbr), const_cast<int32_t*>(S->a.cnt), S->d_bar,
           S->d_rebuilds);
@@ -547,10 +547,10 @@ struct Engine final : EngineBase {
         k_nhc_pre<<<1, 64, 0, s>>>(S->a.nhc, S->nhcp);
         break;
       case OP_NVT_POST:
-        k_nhc_post<false><<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
+        k_nhc_post<false><<<1, 64, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
         break;
       case OP_NVT_POST_PRE:
-        k_nhc_post<true><<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
+        k_nhc_post<true><<<1, 64, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
         break;
       case OP_NVT_FINAL:
         k_scale_v<<<gb, kBlock, 0, s>>>(S->a);
@@ -685,6 +685,33 @@ int sched_alloc(tdmd_sched* S) {
       h_rank[static_cast<size_t>(off + order[static_cast<size_t>(r)].second)] = off + r;
   }
   TDMD_HIP_TRY(hipMemcpy(rank_of, h_rank.data(), h_rank.size() * 4, hipMemcpyHostToDevice));
+  // The list search's stencil: for each cell (row = its rank), the ranks of
+  // the 27 cells around it with periodic wrap, in increasing rank, the order
+  // in which the search visits them (atoms are sorted by cell rank, so rows
+  // come out in increasing atom index).
+  std::vector<int32_t> h_around(27 * static_cast<size_t>(g.ncells));
+  for (int b = 0; b < nbox; ++b) {
+    const int* n = S->multibox ? S->h_boxes[static_cast<size_t>(b)].n : g.n;
+    const int off = S->multibox ? S->h_boxes[static_cast<size_t>(b)].cell_off : 0;
+    for (int z = 0; z < n[2]; ++z)
+      for (int y = 0; y < n[1]; ++y)
+        for (int x = 0; x < n[0]; ++x) {
+          int32_t rk[27];
+          for (int t = 0; t < 27; ++t) {
+            const int nx = (x + t % 3 - 1 + n[0]) % n[0];
+            const int ny = (y + (t / 3) % 3 - 1 + n[1]) % n[1];
+            const int nz = (z + t / 9 - 1 + n[2]) % n[2];
+            rk[t] = h_rank[static_cast<size_t>(off + (nz * n[1] + ny) * n[0] + nx)];
+          }
+          std::sort(rk, rk + 27);
+          const int own = h_rank[static_cast<size_t>(off + (z * n[1] + y) * n[0] + x)];
+          std::copy(rk, rk + 27, h_around.begin() + 27 * static_cast<size_t>(own));
+        }
+  }
+  int32_t* around;
+  if ((rc = dalloc(S, &around, h_around.size()))) return rc;
+  TDMD_HIP_TRY(hipMemcpy(around, h_around.data(), h_around.size() * 4, hipMemcpyHostToDevice));
+  g.around = around;
   if (S->multibox) {
     BoxGeo* boxes;
     int32_t* rep;
@@ -965,7 +992,7 @@ int compute_initial_forces(tdmd_sched* S) {
     // Stored KE of the uploaded velocities (the chain state was reset by
     // upload, with no pending scale).
     k_ke_partials<double><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a);
-    k_nhc_set_ke<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, grid_for(S->natoms), 0);
+    k_nhc_set_ke<<<1, 64, 0, s>>>(S->a.nhc, S->a.partials, grid_for(S->natoms), 0);
     TDMD_LAUNCH_CHECK();
   }
   S->initial_forces = true;

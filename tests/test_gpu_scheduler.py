@@ -174,12 +174,15 @@ def test_displacement_rebuild_policy():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_nvt_matches_oracle_chain(prec):
+@pytest.mark.parametrize("pot,chunks", [("johnson", None), ("johnson", [7, 31, 60]),
+                                        ("morse", None), ("mishin", [25, 60])])
+def test_nvt_matches_oracle_chain(prec, pot, chunks):
+    # windows end with the chain half-step after the step only
     s = fcc(5, T=300.0)
-    say("nvt", prec)
+    say("nvt", prec, pot, chunks)
     cfg = dict(precision=prec, t_target=400.0, t_period=0.05, nhc_length=3)
-    sched, pos, vel, f = gpu_run(s, "johnson", 60, **cfg)
-    rpos, rvel, rf, th, _ = oracle_run(s, "johnson", 60, nvt=True, t_target=400.0,
+    sched, pos, vel, f = gpu_run(s, pot, 60, chunks=chunks, **cfg)
+    rpos, rvel, rf, th, _ = oracle_run(s, pot, 60, nvt=True, t_target=400.0,
                                        t_period=0.05, nhc_length=3)
     ke = I.kinetic_energy(vel, s.types, s.masses)
     say("  ke", ke, th[-1, 2])
@@ -238,6 +241,16 @@ def test_launch_accounting():
     per_step = (b.kernel_launches - a.kernel_launches) / 100.0
     say("  morse launches per step", per_step)
     assert 2.0 <= per_step <= 2.2
+    # EAM NVT: kick-drift, rebuild check, density, force(+kick), chain
+    sched = FastPipelineScheduler.for_system(s, potential("johnson"), FastPipelineConfig(
+        precision="mixed", rebuild_policy="displacement", t_target=300.0))
+    sched.run_until(1)
+    a = sched.stats()
+    sched.run_until(101)
+    b = sched.stats()
+    per_step = (b.kernel_launches - a.kernel_launches) / 100.0
+    say("  nvt launches per step", per_step)
+    assert 5.0 <= per_step <= 5.2
 
 
 def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged():
