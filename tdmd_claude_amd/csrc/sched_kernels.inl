@@ -1092,8 +1092,10 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, int32_t* flags) {
 
 constexpr int kRebuildBlock = 1024;  // workgroup of the fused rebuild
 
+// Registers capped for 8 waves per SIMD: two workgroups per CU (the halo
+// phase's 64 KB of LDS allows two).
 template <class R, class Mir>
-__global__ void __launch_bounds__(kRebuildBlock)
+__global__ void __launch_bounds__(kRebuildBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double near_sq,
                 int32_t* nbr, int32_t* cnt, GridBarrier* bar, int64_t* rebuilds) {
   if (!a.flags[0]) return;  // uniform: nothing writes the flag before the last phase
