@@ -327,6 +327,12 @@ def main() -> int:
             system, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
         ta, tb = nve.time_passes(20)
         log(f"passes: density {ta:.4f} ms, force {tb:.4f} ms")
+        # a list rebuild every 5 steps: the fused rebuild's own cost in a trace
+        reb = FastPipelineScheduler.for_system(
+            system, tables, FastPipelineConfig(precision="mixed", rebuild_policy="fixed",
+                                               rebuild_every=5, mirror_fp64=args.mirror_fp64))
+        wall, _ = timed_run(reb, 0, 20)
+        log(f"rebuild every 5 steps: 20 steps in {wall:.4f} s")
         return 0
 
     # ---- mixed precision (the metric) ----

@@ -191,50 +191,49 @@ __global__ void k_nhc_pre(double* nhc, NhcParams p) {
   nhc[24] = 1.0;
 }
 
-// Sum of n per-block partials by one wavefront, in a fixed order: lane l adds
-// k = l, l + 64, ... in turn, then a butterfly combines the lane sums. IEEE
-// addition commutes, so every lane ends with the same bits, and every caller
-// summing the same partials gets the same result.
-__device__ __forceinline__ double wave_sum_partials(const double* partials, int n) {
-  double t = 0.0;
-  for (int k = static_cast<int>(threadIdx.x & 63); k < n; k += 64) t += partials[k];
-#pragma unroll
-  for (int w = 32; w > 0; w >>= 1) t += __shfl_xor(t, w, 64);
-  return t;
-}
-
-// After the VV step (one thread): half-step on the KE of the kick-2
-// velocities, pending scale = s2. With `pre`, also the next step's
-// half-step on the scaled KE: kick scale = s1 * s2.
-__device__ __forceinline__ void nhc_after_step(double* nhc, double ke, const NhcParams& p,
-                                               bool pre) {
-  const double s2 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
-                                  p.n_dof, ke);
-  nhc[25] = ke;
-  nhc[24] = s2;
-  if (pre) {
-    const double ke1 = s2 * s2 * ke;
-    const double s1 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
-                                    p.n_dof, ke1);
-    nhc[26] = s1 * s2;
-    nhc[24] = 1.0;
-  }
-}
-
-// Chain update after the kick-2 force pass (one wavefront).
+// After the VV step: KE from the per-block partials of the kick-2 epilogue,
+// half-step, pending scale = s2. kPre: the next step's k_nhc_pre follows in
+// the same thread (same operations, one launch less per step).
 template <bool kPre>
 __global__ void k_nhc_post(double* nhc, const double* partials, int n, NhcParams p) {
-  if (threadIdx.x >= 64) return;
-  const double ke = wave_sum_partials(partials, n);
-  if (threadIdx.x == 0) nhc_after_step(nhc, ke, p, kPre);
+  __shared__ double s[256];
+  double t = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) t += partials[k];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double ke = s[0];
+    const double s2 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
+                                    p.n_dof, ke);
+    nhc[25] = ke;
+    nhc[24] = s2;
+    if (kPre) {
+      const double ke1 = nhc[24] * nhc[24] * nhc[25];
+      const double s1 = nhc_half_step(p.M, nhc, nhc + 8, nhc + 16, p.t_target, p.t_period, p.dt,
+                                      p.n_dof, ke1);
+      nhc[26] = s1 * nhc[24];
+      nhc[24] = 1.0;
+    }
+  }
 }
 
 // Stored-KE refresh (after upload, or after the pending scale was applied).
 __global__ void k_nhc_set_ke(double* nhc, const double* partials, int n, int commit_scale) {
-  if (threadIdx.x >= 64) return;
-  const double ke = wave_sum_partials(partials, n);
+  __shared__ double s[256];
+  double t = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) t += partials[k];
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
-    nhc[25] = ke;
+    nhc[25] = s[0];
     if (commit_scale) nhc[24] = 1.0;
   }
 }

@@ -547,10 +547,10 @@ struct Engine final : EngineBase {
         k_nhc_pre<<<1, 64, 0, s>>>(S->a.nhc, S->nhcp);
         break;
       case OP_NVT_POST:
-        k_nhc_post<false><<<1, 64, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
+        k_nhc_post<false><<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
         break;
       case OP_NVT_POST_PRE:
-        k_nhc_post<true><<<1, 64, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
+        k_nhc_post<true><<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, S->nb_force, S->nhcp);
         break;
       case OP_NVT_FINAL:
         k_scale_v<<<gb, kBlock, 0, s>>>(S->a);
@@ -992,7 +992,7 @@ int compute_initial_forces(tdmd_sched* S) {
     // Stored KE of the uploaded velocities (the chain state was reset by
     // upload, with no pending scale).
     k_ke_partials<double><<<grid_for(S->natoms), kBlock, 0, s>>>(S->a);
-    k_nhc_set_ke<<<1, 64, 0, s>>>(S->a.nhc, S->a.partials, grid_for(S->natoms), 0);
+    k_nhc_set_ke<<<1, 256, 0, s>>>(S->a.nhc, S->a.partials, grid_for(S->natoms), 0);
     TDMD_LAUNCH_CHECK();
   }
   S->initial_forces = true;
