@@ -432,6 +432,81 @@ int tdmd_fm_eam(tdmd_fm* fm, int32_t nrho, double drho, int32_t nr,
  * F{a,b,c,d}[nrho-1], rho{a,b,c,d}[nr-1], phi{a,b,c,d}[nr-1]. */
 int tdmd_fm_eam_coef_grad(const tdmd_fm* fm, double* gcoef);
 
+/* ------------------------------------------------------------------ */
+/* Zone pipeline: PipelineScheduler (src/scheduler/                    */
+/* pipeline_scheduler.cuh:26-133, pipeline_scheduler.cu:15-309) with   */
+/* ZonePartition (src/domain/zone_partition.hpp, .cpp:15-143) and the  */
+/* zone state machine (src/scheduler/zone.hpp:17-64). Morse only, as   */
+/* the reference. Slabs along x; atoms sorted by slab at upload; zone   */
+/* steps issued on a pool of n_streams streams (one if deterministic). */
+/* Host arrays: positions / velocities double[3N], forces and masses   */
+/* in `real` of `precision` (float for TDMD_MIXED). download returns   */
+/* the zone order with the ids, as the reference does. Allocates.      */
+/* ------------------------------------------------------------------ */
+typedef struct tdmd_pipe tdmd_pipe;
+
+/* PipelineConfig (pipeline_scheduler.cuh:26-45). */
+typedef struct {
+  double dt;
+  double r_skin;
+  int32_t n_zones;        /* 0: max(3, floor(Lx / rc)) */
+  int32_t n_streams;
+  int32_t rebuild_every;
+  int32_t deterministic;  /* one stream, each zone step waited for */
+  double t_target;        /* > 0: NVT (Nose-Hoover chain) */
+  double t_period;
+  int32_t nhc_length;
+  int32_t adaptive_dt;    /* dt = clamp(c2 rc / vmax, dt_min, dt_max) */
+  double dt_max;
+  double dt_min;
+  double c2;
+  int32_t precision;      /* tdmd_precision */
+  int32_t reserved[7];
+} tdmd_pipe_config;
+
+/* Zone (zone.hpp:43-64): state is the ZoneState code (0 Free .. 6
+ * Sending); lo/hi the slab's bounding box. */
+typedef struct {
+  int32_t id;
+  int32_t atom_offset;
+  int32_t natoms;
+  int32_t time_step;
+  int32_t state;
+  int32_t lattice_index[3];
+  double lo[3];
+  double hi[3];
+} tdmd_zone_info;
+
+/* PipelineScheduler::PipelineScheduler (pipeline_scheduler.cu:15-38). */
+int tdmd_pipe_create(tdmd_pipe** out, const tdmd_box* box, int32_t natoms,
+                     const tdmd_morse_params* params,
+                     const tdmd_pipe_config* cfg);
+int tdmd_pipe_destroy(tdmd_pipe* p);
+/* upload (:40-82): types are 1-based indices into masses[n_masses]. */
+int tdmd_pipe_upload(tdmd_pipe* p, const double* pos, const double* vel,
+                     const void* forces, const int32_t* types,
+                     const int32_t* ids, const void* masses,
+                     int32_t n_masses);
+/* run_until (:196-268): every zone at time_step >= target_step. */
+int tdmd_pipe_run_until(tdmd_pipe* p, int32_t target_step);
+/* download (:279-287), zone order. */
+int tdmd_pipe_download(const tdmd_pipe* p, double* pos, double* vel,
+                       void* forces, int32_t* types, int32_t* ids,
+                       int32_t natoms);
+/* partition() (:74-76): zone count, the zones, and a zone's neighbour
+ * zones (ZonePartition::zone_neighbors; *n gets the full count, at most
+ * cap are written). */
+int tdmd_pipe_zone_count(const tdmd_pipe* p, int32_t* n);
+int tdmd_pipe_zones(const tdmd_pipe* p, tdmd_zone_info* out, int32_t n);
+int tdmd_pipe_zone_neighbors(const tdmd_pipe* p, int32_t zone, int32_t* out,
+                             int32_t cap, int32_t* n);
+/* min_time_step (:270-277); stats (PipelineStats, pipeline_scheduler.cuh:
+ * 48-53): out[0] kernel_launches (zone steps issued), [1] dep_check_calls,
+ * [2] dep_check_failures, [3] ticks; the current (adaptive) dt. */
+int tdmd_pipe_min_time_step(const tdmd_pipe* p, int32_t* out);
+int tdmd_pipe_stats(const tdmd_pipe* p, int64_t out[4]);
+int tdmd_pipe_current_dt(const tdmd_pipe* p, double* dt);
+
 #ifdef __cplusplus
 }
 #endif

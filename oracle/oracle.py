@@ -87,6 +87,8 @@ def lib() -> C.CDLL:
         L.orc_morse_forces.argtypes = [_PD, _PD, _I64, C.POINTER(OrcBox), C.POINTER(OrcNlist),
                                        _D, _D, _D, _D]
         L.orc_morse_forces.restype = _D
+        L.orc_morse_forces_rows.argtypes = [_PD, _PD, _I, _I, _PI, _PI, _PI, C.POINTER(OrcBox),
+                                            _D, _D, _D, _D]
         L.orc_spline_build.argtypes = [C.POINTER(OrcSpline)]
         L.orc_spline_eval.argtypes = [C.POINTER(OrcSpline), _D]
         L.orc_spline_eval.restype = _D
@@ -404,3 +406,105 @@ def run(sys, *, morse=None, eam: Eam | None = None, dt=0.001, skin=1.0, nsteps=0
     if rc != 0:
         raise RuntimeError("orc_run failed")
     return pos, vel, f, th[:out.nthermo].copy(), out.rebuilds
+
+
+# ---------------------------------------------------------------------------
+# Zone pipeline, deterministic mode (PipelineScheduler with
+# cfg.deterministic, src/scheduler/pipeline_scheduler.cu:15-309)
+# ---------------------------------------------------------------------------
+
+def morse_forces_rows(pos, f, first, count, full, lo, hi, morse, periodic=(1, 1, 1)):
+    """compute_morse_gpu_zone (device_morse_zone.cu:29-112), FP64: adds the
+    row sums of atoms [first, first + count) of the full list to f."""
+    counts, offsets, nbrs = full
+    b = box(lo, hi, periodic)
+    D, alpha, r0, rc = morse
+    lib().orc_morse_forces_rows(_pd(pos), _pd(f), int(first), int(count), _pi(counts),
+                                _pi(offsets), _pi(nbrs), C.byref(b), D, alpha, r0, rc)
+
+
+def zone_partition(lo_x, lx, r_cut, n_zones=0):
+    """ZonePartition::build (zone_partition.cpp:15-42): slabs along x,
+    n = max(3, floor(Lx / r_cut)) when not given; returns (n, width)."""
+    if n_zones <= 0:
+        n_zones = max(3, int(np.floor(lx / r_cut)))
+    if n_zones < 3:
+        raise ValueError("need at least 3 zones for PBC")
+    return n_zones, lx / n_zones
+
+
+def zone_neighbors(n_zones, width, r_list):
+    """build_zone_neighbors (zone_partition.cpp:123-143): zones within
+    ceil(r_list / width) slabs, periodic, sorted, unique."""
+    span = int(np.ceil(r_list / width))
+    out = []
+    for z in range(n_zones):
+        v = sorted({(z + d) % n_zones for d in range(-span, span + 1)})
+        out.append(v)
+    return out
+
+
+def assign_atoms(x, lo_x, width, n_zones):
+    """assign_atoms (zone_partition.cpp:51-121): zone_of (:44-49) and a
+    stable counting sort; returns (perm, offsets, counts)."""
+    z = ((np.asarray(x) - lo_x) / width).astype(np.int64)  # truncation, as a C cast
+    z = np.clip(z, 0, n_zones - 1)
+    counts = np.bincount(z, minlength=n_zones)
+    offsets = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    perm = np.argsort(z, kind="stable")
+    return perm, offsets, counts
+
+
+def pipeline_run(sys, morse, *, targets, dt=0.001, skin=1.0, n_zones=0, nvt=None,
+                 adaptive=None):
+    """Deterministic zone pipeline. Each tick sweeps the zones in index
+    order (one stream, each zone step waited for, so every dependency is
+    met): half kick, drift, zero, row forces of the zone over the whole
+    position array, half kick. Forces of all atoms are recomputed at the
+    start of every run_until (`targets` lists the run_until calls); the
+    list is built from the uploaded positions and never again (the
+    reference's rebuild flag is set by upload only). nvt = (t_target,
+    t_period, M): chain half-steps on the global KE around every tick.
+    adaptive = (dt_min, dt_max, c2): dt = clamp(c2 rc / vmax) before every
+    tick. Returns (pos, vel, forces, types, ids, zone offsets, counts), zone
+    order."""
+    lo = np.asarray(sys.lo, dtype=np.float64)
+    hi = np.asarray(sys.hi, dtype=np.float64)
+    D, alpha, r0, rc = morse
+    nz, width = zone_partition(lo[0], hi[0] - lo[0], rc, n_zones)
+    perm, offsets, counts = assign_atoms(sys.pos[:, 0], lo[0], width, nz)
+    pos = np.ascontiguousarray(sys.pos[perm], dtype=np.float64)
+    vel = np.ascontiguousarray(sys.vel[perm], dtype=np.float64)
+    types = np.ascontiguousarray(sys.types[perm], dtype=np.int32)
+    ids = np.ascontiguousarray(sys.ids[perm], dtype=np.int32)
+    masses = np.ascontiguousarray(sys.masses, dtype=np.float64)
+    n = len(pos)
+    half = HalfList(pos, lo, hi, rc, skin, sys.periodic)
+    full = half.full()
+    f = np.zeros_like(pos)
+    chain = Nhc(nvt[0], nvt[1], nvt[2], dt, 3 * n - 3) if nvt else None
+    step = 0
+    for target in targets:
+        f[:] = 0.0
+        morse_forces_rows(pos, f, 0, n, full, lo, hi, morse, sys.periodic)
+        while step < target:
+            if adaptive:
+                vmax = float(np.sqrt(np.max(np.sum(vel * vel, axis=1))))
+                if vmax > 0:
+                    dt = max(min(adaptive[1], adaptive[2] * rc / vmax), adaptive[0])
+            if chain:
+                vel *= chain.half_step(kinetic_energy(vel, types, masses))
+            for z in range(nz):
+                a, c = int(offsets[z]), int(counts[z])
+                if c == 0:
+                    continue
+                sl = slice(a, a + c)
+                half_kick(vel[sl], f[sl], types[sl], masses, dt)
+                drift(pos[sl], vel[sl], lo, hi, dt, single_image=True, periodic=sys.periodic)
+                f[sl] = 0.0
+                morse_forces_rows(pos, f, a, c, full, lo, hi, morse, sys.periodic)
+                half_kick(vel[sl], f[sl], types[sl], masses, dt)
+            if chain:
+                vel *= chain.half_step(kinetic_energy(vel, types, masses))
+            step += 1
+    return pos, vel, f, types, ids, offsets, counts

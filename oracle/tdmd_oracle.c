@@ -340,6 +340,38 @@ double orc_morse_forces(const double* pos, double* f, int64_t natoms,
   return total;
 }
 
+void orc_morse_forces_rows(const double* pos, double* f, int first, int count,
+                           const int* counts, const int* offsets, const int* nbrs,
+                           const orc_box* box, double D, double alpha, double r0,
+                           double rc) {
+  /* morse_force_zone_kernel (device_morse_zone.cu:29-112) in FP64: each row
+   * i of [first, first + count) of a full list, d = x_i - x_j with one
+   * minimum image, F_i += fpair d summed in row order, then ADDED to f. */
+  const double rc_sq = rc * rc;
+  double bs[3];
+  box_size(box, bs);
+  for (int i = first; i < first + count; ++i) {
+    const double* pi = pos + 3 * (int64_t)i;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int* nb = nbrs + offsets[i];
+    for (int k = 0; k < counts[i]; ++k) {
+      const int j = nb[k];
+      double d[3] = {pi[0] - pos[3 * (int64_t)j], pi[1] - pos[3 * (int64_t)j + 1],
+                     pi[2] - pos[3 * (int64_t)j + 2]};
+      orc_minimum_image(d, bs, box->periodic);
+      const double r2 = length_sq(d);
+      double e, fp;
+      orc_morse_pair(D, alpha, r0, rc_sq, r2, &e, &fp);
+      fx += fp * d[0];
+      fy += fp * d[1];
+      fz += fp * d[2];
+    }
+    f[3 * (int64_t)i] += fx;
+    f[3 * (int64_t)i + 1] += fy;
+    f[3 * (int64_t)i + 2] += fz;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* FD-Hermite spline: src/potentials/eam_alloy.cpp:19-72                    */
 /* ------------------------------------------------------------------------ */

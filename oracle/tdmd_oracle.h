@@ -101,6 +101,12 @@ void orc_morse_pair(double D, double alpha, double r0, double rc_sq,
 double orc_morse_forces(const double* pos, double* forces, int64_t natoms,
                         const orc_box* box, const orc_nlist* nl, double D,
                         double alpha, double r0, double rc);
+/* compute_morse_gpu_zone (device_morse_zone.cu:29-112), FP64: rows
+ * [first, first + count) of a full list; forces ADDED to f. */
+void orc_morse_forces_rows(const double* pos, double* f, int first, int count,
+                           const int* counts, const int* offsets, const int* nbrs,
+                           const orc_box* box, double D, double alpha, double r0,
+                           double rc);
 
 /* ---- spline: src/potentials/eam_alloy.cpp:19-72 ---- */
 typedef struct {

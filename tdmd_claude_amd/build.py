@@ -30,6 +30,7 @@ SOURCES = [
     "integrate.cpp",
     "scheduler.cpp",
     "fm.cpp",
+    "pipeline.cpp",
 ]
 
 ARCH = "gfx950"

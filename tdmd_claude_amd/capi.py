@@ -111,6 +111,43 @@ class Observables(C.Structure):
     ]
 
 
+class PipeConfig(C.Structure):
+    """tdmd_pipe_config (PipelineConfig)."""
+
+    _fields_ = [
+        ("dt", C.c_double),
+        ("r_skin", C.c_double),
+        ("n_zones", C.c_int32),
+        ("n_streams", C.c_int32),
+        ("rebuild_every", C.c_int32),
+        ("deterministic", C.c_int32),
+        ("t_target", C.c_double),
+        ("t_period", C.c_double),
+        ("nhc_length", C.c_int32),
+        ("adaptive_dt", C.c_int32),
+        ("dt_max", C.c_double),
+        ("dt_min", C.c_double),
+        ("c2", C.c_double),
+        ("precision", C.c_int32),
+        ("reserved", C.c_int32 * 7),
+    ]
+
+
+class ZoneInfo(C.Structure):
+    """tdmd_zone_info (Zone)."""
+
+    _fields_ = [
+        ("id", C.c_int32),
+        ("atom_offset", C.c_int32),
+        ("natoms", C.c_int32),
+        ("time_step", C.c_int32),
+        ("state", C.c_int32),
+        ("lattice_index", C.c_int32 * 3),
+        ("lo", C.c_double * 3),
+        ("hi", C.c_double * 3),
+    ]
+
+
 _VP = C.c_void_p
 _I32 = C.c_int32
 _I64 = C.c_int64
@@ -202,6 +239,18 @@ _PROTOS = {
     "tdmd_sched_halo_info": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
     "tdmd_sched_time_passes": (C.c_int, [_VP, _I32, C.POINTER(C.c_float)]),
     "tdmd_sched_autotune": (C.c_int, [_VP, _I32, _PI32]),
+    "tdmd_pipe_create": (C.c_int, [C.POINTER(_VP), C.POINTER(Box), _I32, C.POINTER(MorseParams),
+                                   C.POINTER(PipeConfig)]),
+    "tdmd_pipe_destroy": (C.c_int, [_VP]),
+    "tdmd_pipe_upload": (C.c_int, [_VP, _PD, _PD, _VP, _PI32, _PI32, _VP, _I32]),
+    "tdmd_pipe_run_until": (C.c_int, [_VP, _I32]),
+    "tdmd_pipe_download": (C.c_int, [_VP, _PD, _PD, _VP, _PI32, _PI32, _I32]),
+    "tdmd_pipe_zone_count": (C.c_int, [_VP, _PI32]),
+    "tdmd_pipe_zones": (C.c_int, [_VP, C.POINTER(ZoneInfo), _I32]),
+    "tdmd_pipe_zone_neighbors": (C.c_int, [_VP, _I32, _PI32, _I32, _PI32]),
+    "tdmd_pipe_min_time_step": (C.c_int, [_VP, _PI32]),
+    "tdmd_pipe_stats": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
+    "tdmd_pipe_current_dt": (C.c_int, [_VP, _PD]),
     "tdmd_fm_create": (C.c_int, [C.POINTER(_VP), _I32, C.POINTER(Box), _I32, _I32, _PD, _D]),
     "tdmd_fm_destroy": (C.c_int, [_VP]),
     "tdmd_fm_info": (C.c_int, [_VP, _PI32, C.POINTER(_I64)]),
