@@ -45,4 +45,10 @@ using ForceVec = Vec3T<force_t>;
 
 static_assert(sizeof(Vec3D) == 24 && sizeof(Vec3F) == 12, "Vec3 must be packed AoS");
 
+// Axis-aligned box (src/core/types.hpp:55-58): a zone's slab.
+struct Aabb {
+  Vec3D lo;
+  Vec3D hi;
+};
+
 }  // namespace tdmd

@@ -17,4 +17,7 @@
 #include "potentials/device_morse.hpp"
 #include "potentials/device_morse_zone.hpp"
 #include "potentials/setfl_potential.hpp"
+#include "domain/zone_partition.hpp"
 #include "scheduler/fast_pipeline_scheduler.hpp"
+#include "scheduler/pipeline_scheduler.hpp"
+#include "scheduler/zone.hpp"

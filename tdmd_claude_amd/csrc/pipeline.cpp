@@ -129,9 +129,9 @@ struct tdmd_pipe {
   size_t real_bytes() const { return mixed ? 4 : 8; }
   int32_t prec() const { return mixed ? TDMD_MIXED : TDMD_FP64; }
 
-  // ZonePartition::zone_of (zone_partition.cpp:44-49)
+  // ZonePartition::zone_of (zone_partition.cpp:44-49); x arrives as `real`
   int zone_of(double x) const {
-    int z = static_cast<int>((x - lo_x) / zone_width);
+    int z = static_cast<int>(rl(rl(rl(x) - lo_x) / zone_width));
     if (z < 0) z = 0;
     if (z >= nz) z = nz - 1;
     return z;

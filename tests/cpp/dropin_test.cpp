@@ -1,7 +1,7 @@
 // dropin_test.cpp -- the C++ drop-in interface (include/tdmd/tdmd.hpp)
 // exercised the way the reference's own code calls it: DeviceNeighborList,
 // compute_morse_gpu, DeviceEam, the integrator functions, the NHC
-// reductions and FastPipelineScheduler. Checks against host brute force and
+// reductions, FastPipelineScheduler and the zone pipeline. Checks against host brute force and
 // against the reference's step sequence assembled from the free functions
 // (fast_pipeline_scheduler.cu:101-166,195-232). Built twice, once per
 // precision macro. Prints one line per check; exit status 0 = all passed.
@@ -162,7 +162,8 @@ double max_rel(const std::vector<Vec3D>& a, const std::vector<ForceVec>& b) {
 }
 
 potentials::MorseParams morse_c1() {
-  // SURVEY C1 parameters: D 0.3429 eV, alpha 1.3588 1/A, r0 2.866 A, rc 6.0 A.
+  // Cu Morse of the reference tests (tests/unit/test_pipeline_scheduler.cu:52):
+  // D 0.3429 eV, alpha 1.3588 1/A, r0 2.866 A, rc 6.0 A.
   return potentials::MorseParams{real(0.3429), real(1.3588), real(2.866), real(6.0), real(36.0)};
 }
 
@@ -271,6 +272,123 @@ void check_scheduler_morse(const System& s) {
          "max |dx| " + std::to_string(dx) + " A, max |dv| " + std::to_string(dv) +
              ", rebuilds " + std::to_string(st.rebuilds) + ", launches " +
              std::to_string(st.kernel_launches));
+}
+
+// The zone pipeline: ZonePartition against the reference's partition tests
+// (tests/unit/test_zone_partition.cpp), the deterministic PipelineScheduler
+// against the zone sweep assembled from the drop-in free functions (the
+// zone step of pipeline_scheduler.cu:128-168 in index order), and the
+// pipelined mode's zones all reaching the target
+// (tests/unit/test_pipeline_scheduler.cu:184-216).
+void check_pipeline(const System& s) {
+  const int N = static_cast<int>(s.pos.size());
+  const auto p = morse_c1();
+  {
+    const System lat = fcc(4, 3.615, 0.0, 0.0, 1);  // the reference's cu_fcc_256 lattice
+    domain::ZonePartition zp;
+    zp.build(lat.box, real(6.0));
+    auto x = lat.pos;
+    auto v = lat.vel;
+    std::vector<ForceVec> f(N, ForceVec{0, 0, 0});
+    auto ty = lat.types;
+    auto id = lat.ids;
+    zp.assign_atoms(x.data(), v.data(), f.data(), ty.data(), id.data(), N);
+    std::set<i32> seen(id.begin(), id.end());
+    bool inside = true;
+    int total = 0;
+    for (const auto& z : zp.zones()) {
+      total += z.natoms_in_zone;
+      for (int k = z.atom_offset; k < z.atom_offset + z.natoms_in_zone; ++k)
+        inside = inside && x[k].x >= z.bbox.lo.x && x[k].x < z.bbox.hi.x + 1e-10;
+    }
+    Box b20;
+    b20.hi = Vec3D{20, 20, 20};
+    domain::ZonePartition z4;
+    z4.build(b20, real(5.0), 4);
+    z4.build_zone_neighbors(real(6.5));
+    const bool all4 = z4.zone_neighbors(0).size() == 4 && z4.zone_neighbors(3).size() == 4;
+    z4.build_zone_neighbors(real(4.0));
+    const bool three = z4.zone_neighbors(1).size() == 3 && z4.zone_neighbors(1)[0] == 0;
+    bool threw = false;
+    scheduler::Zone zone;
+    try {
+      zone.transition_to(scheduler::ZoneState::Computing);  // Free -> Computing
+    } catch (const Error& e) {
+      threw = e.status() == TDMD_E_STATE;
+    }
+    report("ZonePartition / zone state machine",
+           zp.n_zones() == 3 && total == N && static_cast<int>(seen.size()) == N && inside &&
+               all4 && three && threw,
+           std::to_string(zp.n_zones()) + " zones of " + std::to_string(zp.zone_width()) + " A");
+  }
+  const real m[2] = {real(1.0), real(63.546)};
+  const std::vector<ForceVec> f0(N, ForceVec{0, 0, 0});
+  scheduler::PipelineConfig cfg;
+  cfg.deterministic = true;
+  scheduler::PipelineScheduler pipe(s.box, N, p, cfg);
+  pipe.upload(s.pos.data(), s.vel.data(), f0.data(), s.types.data(), s.ids.data(), m, 2);
+  const int nsteps = 40;
+  pipe.run_until(nsteps);
+  std::vector<PositionVec> x(N);
+  std::vector<VelocityVec> v(N);
+  std::vector<ForceVec> f(N);
+  std::vector<i32> ty(N), id(N);
+  pipe.download(x.data(), v.data(), f.data(), ty.data(), id.data(), N);
+  // the sweep from the free functions, on the same zone order
+  domain::ZonePartition zp;
+  zp.build(s.box, p.rc);
+  auto xr = s.pos;
+  auto vr = s.vel;
+  auto fr = f0;
+  auto tr = s.types;
+  auto ir = s.ids;
+  zp.assign_atoms(xr.data(), vr.data(), fr.data(), tr.data(), ir.data(), N);
+  DeviceBuffer<PositionVec> dx(N);
+  DeviceBuffer<VelocityVec> dv(N);
+  DeviceBuffer<ForceVec> df(N);
+  DeviceBuffer<i32> dt(N);
+  DeviceBuffer<real> dm(2);
+  dx.copy_from_host(xr.data(), N);
+  dv.copy_from_host(vr.data(), N);
+  dt.copy_from_host(tr.data(), N);
+  dm.copy_from_host(m, 2);
+  neighbors::DeviceNeighborList nl;
+  nl.build(dx.data(), N, s.box, p.rc, cfg.r_skin);
+  df.zero();
+  potentials::compute_morse_gpu(dx.data(), df.data(), nl.d_neighbors(), nl.d_offsets(),
+                                nl.d_counts(), N, s.box, p, nullptr);
+  for (int step = 0; step < nsteps; ++step)
+    for (const auto& z : zp.zones()) {
+      const i32 a = z.atom_offset, c = z.natoms_in_zone;
+      integrator::device_half_kick_zone(dv.data(), df.data(), dt.data(), dm.data(), a, c, cfg.dt);
+      integrator::device_drift_zone(dx.data(), dv.data(), a, c, cfg.dt, s.box);
+      integrator::device_zero_forces_zone(df.data(), a, c);
+      potentials::compute_morse_gpu_zone(dx.data(), df.data(), nl.d_neighbors(), nl.d_offsets(),
+                                         nl.d_counts(), a, c, s.box, p, nullptr);
+      integrator::device_half_kick_zone(dv.data(), df.data(), dt.data(), dm.data(), a, c, cfg.dt);
+    }
+  dx.copy_to_host(xr.data(), N);
+  dv.copy_to_host(vr.data(), N);
+  bool same = true;
+  for (int i = 0; i < N; ++i)
+    same = same && x[i].x == xr[i].x && x[i].y == xr[i].y && x[i].z == xr[i].z &&
+           v[i].x == vr[i].x && v[i].y == vr[i].y && v[i].z == vr[i].z && id[i] == ir[i];
+  const auto& st = pipe.stats();
+  report("PipelineScheduler(deterministic) vs zone sweep of the free functions",
+         same && pipe.min_time_step() == nsteps &&
+             st.kernel_launches == static_cast<i64>(nsteps) * pipe.partition().n_zones(),
+         std::string(same ? "bitwise equal" : "differ") + ", " +
+             std::to_string(pipe.partition().n_zones()) + " zones, " +
+             std::to_string(st.ticks) + " ticks");
+  scheduler::PipelineConfig c2;
+  c2.n_streams = 2;
+  scheduler::PipelineScheduler pipe2(s.box, N, p, c2);
+  pipe2.upload(s.pos.data(), s.vel.data(), f0.data(), s.types.data(), s.ids.data(), m, 2);
+  pipe2.run_until(10);
+  bool all = pipe2.min_time_step() >= 10;
+  for (const auto& z : pipe2.partition().zones()) all = all && z.time_step >= 10;
+  report("PipelineScheduler(2 streams): zones reach the target", all,
+         "min time step " + std::to_string(pipe2.min_time_step()));
 }
 
 void check_eam(const System& s, const std::string& setfl) {
@@ -448,6 +566,7 @@ int main(int argc, char** argv) {
   check_neighbor_list(s);
   check_morse(s);
   check_scheduler_morse(s);
+  check_pipeline(s);
   check_eam(s, argv[1]);
   check_nhc_reductions(s);
   check_errors();

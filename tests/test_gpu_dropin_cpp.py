@@ -26,7 +26,7 @@ def test_cpp_dropin_program(prec, tmp_path):
     print(res.stdout, res.stderr, flush=True)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "0 failure(s)" in res.stdout
-    assert res.stdout.count("PASS") == 8
+    assert res.stdout.count("PASS") == 11
 
 
 def test_cpp_dropin_program_is_built():
