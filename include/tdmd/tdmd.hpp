@@ -4,6 +4,7 @@
 #pragma once
 
 #include "core/box.hpp"
+#include "core/system_state.hpp"
 #include "core/device_buffer.hpp"
 #include "core/types.hpp"
 #include "domain/device_cell_list.hpp"
@@ -11,6 +12,8 @@
 #include "integrator/device_velocity_verlet.hpp"
 #include "integrator/device_velocity_verlet_zone.hpp"
 #include "io/dump.hpp"
+#include "io/lammps_data.hpp"
+#include "io/setfl.hpp"
 #include "io/thermo.hpp"
 #include "neighbors/device_neighbor_list.hpp"
 #include "potentials/device_eam.hpp"

@@ -116,3 +116,102 @@ def test_malformed_dumps_are_errors(tmp_path, body, msg):
         OUT.read_dump(p)
     r = subprocess.run([io_bin("fp64"), "read", str(p)], capture_output=True, text=True)
     assert r.returncode == 2 and msg in r.stdout
+
+
+# ---------------------------------------------------------------------------
+# LAMMPS data files and setfl tables (lammps_data_reader.cpp:33-205,
+# eam_alloy.cpp:90-192): the C++ headers against the Python reader/writers
+# and the reference's own data files.
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["cu_fcc_256.data", "cu_fcc_32.data", "4atoms.data"])
+def test_cpp_data_reader_writer_equal_python(tmp_path, name):
+    src = GOLDEN / name
+    ref = I.read_lammps_data(src)
+    for prec in ("fp64", "mixed"):
+        out = tmp_path / f"{prec}.data"
+        res = subprocess.run([io_bin(prec), "data_copy", str(src), str(out)], capture_output=True,
+                             text=True)
+        assert res.returncode == 0, res.stdout + res.stderr
+        assert int(res.stdout) == ref.natoms
+        back = I.read_lammps_data(out)
+        assert np.array_equal(back.ids, ref.ids) and np.array_equal(back.types, ref.types)
+        assert np.array_equal(back.lo, ref.lo) and np.array_equal(back.hi, ref.hi)
+        if prec == "fp64":
+            py = tmp_path / "py.data"
+            I.write_lammps_data(py, ref)
+            assert out.read_bytes() == py.read_bytes()
+        else:  # numbers parsed as float, as the reference's mixed build does
+            assert np.abs(back.pos - ref.pos).max() <= 1e-6 * max(1.0, np.abs(ref.pos).max())
+            assert np.allclose(back.masses, ref.masses, rtol=1e-7)
+
+
+def test_cpp_data_velocities_round_trip(tmp_path):
+    s = I.fcc_lattice(2, 2, 2, 3.615)
+    I.init_velocities(s, 500.0, 9)
+    src = tmp_path / "v.data"
+    I.write_lammps_data(src, s)
+    out = tmp_path / "c.data"
+    res = subprocess.run([io_bin("fp64"), "data_copy", str(src), str(out)], capture_output=True,
+                         text=True)
+    assert res.returncode == 0, res.stdout
+    assert out.read_bytes() == src.read_bytes()
+    back = I.read_lammps_data(out)
+    assert np.array_equal(back.vel, s.vel)
+
+
+@pytest.mark.parametrize("broken,msg", [
+    ("drop_box", "incomplete box definition"),
+    ("drop_count", "no atoms in data file"),
+    ("short_atoms", "premature end in Atoms"),
+    ("bad_id", "atom id outside 1..N"),
+    ("bad_row", "bad Atoms line"),
+])
+def test_cpp_data_reader_errors(tmp_path, broken, msg):
+    lines = (GOLDEN / "4atoms.data").read_text().splitlines()
+    if broken == "drop_box":
+        lines = [ln for ln in lines if "ylo yhi" not in ln]
+    elif broken == "drop_count":
+        lines = [ln for ln in lines if not ln.strip().endswith(" atoms")]
+    elif broken == "short_atoms":
+        k = max(i for i, ln in enumerate(lines) if ln.strip() and ln.split()[0].isdigit()
+                and len(ln.split()) == 5)
+        lines = lines[:k]
+        # no later section: the file ends inside Atoms
+        lines = [ln for ln in lines]
+    elif broken == "bad_id":
+        k = next(i for i, ln in enumerate(lines) if ln.strip().startswith("Atoms")) + 2
+        f = lines[k].split()
+        f[0] = "99"
+        lines[k] = " ".join(f)
+    elif broken == "bad_row":
+        k = next(i for i, ln in enumerate(lines) if ln.strip().startswith("Atoms")) + 2
+        lines[k] = lines[k].split()[0] + " 1 x y z"
+    p = tmp_path / "broken.data"
+    p.write_text("\n".join(lines) + "\n")
+    res = subprocess.run([io_bin("fp64"), "data_copy", str(p), str(tmp_path / "o.data")],
+                         capture_output=True, text=True)
+    assert res.returncode == 2 and msg in res.stdout, res.stdout
+
+
+def test_cpp_setfl_writer_equals_python(tmp_path):
+    t = I.johnson_alloy_setfl(nr=120, nrho=90)
+    tab = tmp_path / "tables.txt"
+    with open(tab, "w") as fh:
+        fh.write(f"{t.ntypes} {t.nrho} {t.drho!r} {t.nr} {t.dr!r} {t.cutoff!r}\n")
+        fh.write(" ".join(t.elements) + "\n")
+        fh.write(" ".join(repr(float(m)) for m in t.masses) + "\n")
+        for e in range(t.ntypes):
+            fh.write(" ".join(repr(float(x)) for x in t.F[e]) + "\n")
+            fh.write(" ".join(repr(float(x)) for x in t.rho[e]) + "\n")
+        for p in range(t.rphi.shape[0]):
+            fh.write(" ".join(repr(float(x)) for x in t.rphi[p]) + "\n")
+    out = tmp_path / "c.setfl"
+    res = subprocess.run([io_bin("fp64"), "setfl_write", str(tab), str(out)], capture_output=True,
+                         text=True)
+    assert res.returncode == 0, res.stdout + res.stderr
+    py = tmp_path / "py.setfl"
+    I.write_setfl(py, t)
+    assert out.read_bytes() == py.read_bytes()
+    back = I.read_setfl(out)
+    assert back.elements == list(t.elements) and np.allclose(back.rphi, t.rphi, rtol=1e-15)
