@@ -51,6 +51,36 @@ __device__ __forceinline__ PairR<float> pair_r<float>(double r2) {
   return p;
 }
 
+// Density pairs under mixed precision: r, the interval and the offset in
+// FP32 from the squared distance rounded to float. The density pass's pair
+// values reach the forces only through rho_i, a sum over ~60 pairs: an
+// offset error of ~4e-7 A (float r, float interval coordinate) moves rho_i
+// by ~1e-7 relative, far inside the force tolerance, and saves the FP64
+// Newton step and FP64 interval arithmetic on every pair. The force pass
+// keeps the FP64 r, since its pair terms scale the FP64 difference vector.
+struct SplineMetaF {
+  float inv_dr;
+  float off;  // rmin / dr
+  float dr;
+  int last;   // n - 2
+};
+
+__device__ __forceinline__ SplineMetaF meta_f32(const SplineMeta& m) {
+  return SplineMetaF{static_cast<float>(m.inv_dr), static_cast<float>(m.rmin * m.inv_dr),
+                     static_cast<float>(m.dr), m.n - 2};
+}
+
+// Interval of the pair at squared distance r2 and the offset into it.
+__device__ __forceinline__ int locate_f32(const SplineMetaF& m, double r2, float* dx) {
+  const float r = __builtin_amdgcn_sqrtf(static_cast<float>(r2));
+  const float u = fmaf(r, m.inv_dr, -m.off);
+  int idx = static_cast<int>(u);
+  idx = idx < 0 ? 0 : idx;
+  idx = idx > m.last ? m.last : idx;
+  *dx = (u - static_cast<float>(idx)) * m.dr;
+  return idx;
+}
+
 template <class R>
 struct SplineVal {
   R f, df;

@@ -402,8 +402,14 @@ k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
-        const PairR<R> pr = pair_r<R>(r2);
-        rho += static_cast<double>(spline_value<R>(sm[a.ntypes + Mir::type_of(mj)], coeff, pr.r));
+        const SplineMeta& m = sm[a.ntypes + Mir::type_of(mj)];
+        if constexpr (std::is_same_v<R, float>) {
+          float t;
+          const auto c = coeff[m.offset + locate_f32(meta_f32(m), r2, &t)];
+          rho += static_cast<double>(c.x + t * (c.y + t * (c.z + t * c.w)));
+        } else {
+          rho += spline_value<R>(m, coeff, pair_r<R>(r2).r);
+        }
       }
     });
     const SplineVal<R> F = spline_both<R>(sm[ti], coeff, rho);

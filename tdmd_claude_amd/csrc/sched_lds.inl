@@ -184,6 +184,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   extern __shared__ __align__(16) unsigned char smem[];
   Q* tab = reinterpret_cast<Q*>(smem);
   const SplineMeta mF = meta[0], mrho = meta[1];
+  const SplineMetaF mrho_f = meta_f32(mrho);
   for (int q = threadIdx.x; q < w.nwin; q += kBigBlock) tab[q] = coeff[mrho.offset + w.k_lo + q];
   const T* __restrict__ M = static_cast<const T*>(a.M);
   T* hrec = reinterpret_cast<T*>(smem + window_bytes(w.nwin, sizeof(Q)));
@@ -214,9 +215,13 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
       double dx, dy, dz;
       const double s2 = pair_delta<Mir, kIso>(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && s2 < rcut2) {
-        const PairR<R> pr = pair_r<R>(kIso ? s2 * u2 : s2);
         R t;
-        const int idx = spline_locate<R>(mrho, pr.r, &t);
+        int idx;
+        if constexpr (std::is_same_v<R, float>) {
+          idx = locate_f32(mrho_f, kIso ? s2 * u2 : s2, &t);
+        } else {
+          idx = spline_locate<R>(mrho, pair_r<R>(kIso ? s2 * u2 : s2).r, &t);
+        }
         const int q = idx - w.k_lo;
         const Q cf = (!kWin || q >= 0) ? tab[q] : coeff[mrho.offset + idx];
         rho += static_cast<double>(cubic_value<R>(cf.x, cf.y, cf.z, cf.w, t));
