@@ -474,21 +474,54 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
 // Morse pass (morse.hpp:37-55; pair_math.hpp precision contract)
 // ---------------------------------------------------------------------------
 
-template <class R, class Mir, int kEpi, bool kEnergy>
+// A row split over kLanes neighbouring lanes (small systems, where one lane
+// per atom leaves most of the chip idle): lane `part` takes quads part,
+// part + kLanes, ... of the padded row; the four records of a quad are
+// gathered before its pairs are evaluated.
+template <class T, int kLanes, class Pair>
+__device__ __forceinline__ void for_neighbours_split(const StepArgs& a, int i, int n, int part,
+                                                     const T* __restrict__ rec, Pair&& pair) {
+  const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
+  const int nq = (n + 3) >> 2;  // rows are padded to 8 slots: whole quads
+  for (int q = part; q < nq; q += kLanes) {
+    const int4 s = q4[ell(i, 4 * q, a.cap) >> 2];
+    const T r0 = gather(rec, s.x), r1 = gather(rec, s.y), r2 = gather(rec, s.z),
+            r3 = gather(rec, s.w);
+    const int k = 4 * q;
+    pair(s.x, r0, k < n);
+    pair(s.y, r1, k + 1 < n);
+    pair(s.z, r2, k + 2 < n);
+    pair(s.w, r3, k + 3 < n);
+  }
+}
+
+// Sum over the kLanes lanes of one atom (butterfly; every lane ends with
+// the same bits, so the order is fixed).
+template <int kLanes>
+__device__ __forceinline__ double lane_sum(double v) {
+#pragma unroll
+  for (int w = kLanes / 2; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
+  return v;
+}
+
+template <class R, class Mir, int kEpi, bool kEnergy, int kLanes>
 __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p) {
   // The pass gathers from M; with the fused kick-drift the next step's
   // records go to M2 and the host swaps the two buffers after the launch.
+  // kLanes > 1: kLanes lanes per atom, the grid kLanes times larger.
+  static_assert(kLanes >= 1 && 64 % kLanes == 0, "an atom's lanes share a wavefront");
   using T = typename Mir::T;
-  const int blk = xcd_block(a.nblocks);
-  const int i = blk * kBlock + threadIdx.x;
+  const int blk = xcd_block(gridDim.x);
+  const int g = blk * kBlock + threadIdx.x;
+  const int i = g / kLanes;
+  const int part = g % kLanes;
   const T* __restrict__ M = static_cast<const T*>(a.M);
-  double red = 0.0, vir = 0.0;
+  double red = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0, vir = 0.0;
   if (i < a.natoms) {
     const T mi = M[i];
     const MirrorParams& mpi = mp_of<Mir>(a, i);
     const int n = a.cnt[i];
-    double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-    for_neighbours<T, kPairUnroll>(a, i, n, M, [&](int, const T& mj, bool in_row) {
+    auto pair = [&](int, const T& mj, bool in_row) {
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < p.rc_sq) {
@@ -503,9 +536,26 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
         fy = fma(fpair, dy, fy);
         fz = fma(fpair, dz, fz);
       }
-    });
+    };
+    if constexpr (kLanes == 1)
+      for_neighbours<T, kPairUnroll>(a, i, n, M, pair);
+    else
+      for_neighbours_split<T, kLanes>(a, i, n, part, M, pair);
+  }
+  if constexpr (kLanes > 1) {  // every lane takes part in the shuffles
+    fx = lane_sum<kLanes>(fx);
+    fy = lane_sum<kLanes>(fy);
+    fz = lane_sum<kLanes>(fz);
+    if (kEnergy) {
+      pe = lane_sum<kLanes>(pe);
+      vir = lane_sum<kLanes>(vir);
+    }
+  }
+  if (i < a.natoms && part == 0) {
     const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz, a.M2);
     red = kEnergy ? pe : ke;
+  } else {
+    vir = 0.0;
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
   if (kEnergy) emit_partial(vir, a.partials_w, blk);

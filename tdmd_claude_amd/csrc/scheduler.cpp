@@ -39,6 +39,12 @@ namespace {
 
 using namespace sched;
 
+// Morse rows of systems up to kMorseSplitMax atoms are split over
+// kMorseLanes lanes: one lane per atom would give 32,000 atoms 125
+// workgroups on 256 CUs, each lane walking ~80 slots in turn.
+constexpr int kMorseLanes = 8;
+constexpr int kMorseSplitMax = 131072;
+
 // Operation codes of a step sequence (also the graph-cache key alphabet).
 enum Op : char {
   OP_BOOT = 'b',        // half kick + drift (window start, NVE)
@@ -99,6 +105,7 @@ struct tdmd_sched {
   int natoms = 0;
   int nblocks = 0;   // 256-thread grid (multiple of 8)
   int nb_force = 0;  // grid of the EAM/Morse passes = block partials they emit
+  int morse_lanes = 1;  // lanes per atom of the Morse pass
   int nb_rebuild = 0;  // fused rebuild grid (all resident at once); 0: separate kernels
   tdmd_hip::sched::GridBarrier* d_bar = nullptr;
   tdmd_sched_config cfg{};
@@ -242,7 +249,10 @@ struct Engine final : EngineBase {
   }
 
   int init() override {
-    S->nb_force = S->nblocks;
+    // Morse rows split over kMorseLanes lanes when one lane per atom would
+    // leave the chip mostly idle (partials: one per block of the larger grid)
+    S->morse_lanes = S->kind == 0 && S->natoms <= kMorseSplitMax ? kMorseLanes : 1;
+    S->nb_force = S->nblocks * S->morse_lanes;
     S->nb_rebuild = 0;
     if (!S->cfg.split_rebuild) {
       // every workgroup of the fused rebuild must be resident at once
@@ -451,7 +461,11 @@ struct Engine final : EngineBase {
         go(k_eam_force<R, Mir, kEpi, kEnergy, false>, nb, kBlock, 0, s, S->a, meta(), coeff(),
            rc_sq());
     } else {
-      go(k_morse_force<R, Mir, kEpi, kEnergy>, nb, kBlock, 0, s, S->a, mp());
+      if (S->morse_lanes > 1)
+        go(k_morse_force<R, Mir, kEpi, kEnergy, kMorseLanes>, nb * kMorseLanes, kBlock, 0, s,
+           S->a, mp());
+      else
+        go(k_morse_force<R, Mir, kEpi, kEnergy, 1>, nb, kBlock, 0, s, S->a, mp());
       if constexpr (kEpi == kFullKickDrift) {
         // the epilogue wrote the next step's records into M2: it is now M
         std::swap(S->a.M, S->a.M2);
@@ -630,8 +644,10 @@ int sched_alloc(tdmd_sched* S) {
   a.M = M;
   a.M2 = M2;
   if ((rc = dalloc(S, &a.flags, 8))) return rc;
-  if ((rc = dalloc(S, &a.partials, S->nblocks + 64))) return rc;
-  if ((rc = dalloc(S, &a.partials_w, S->nblocks + 64))) return rc;
+  // room for the per-block partials of the widest pass (Morse rows split
+  // over kMorseLanes lanes)
+  if ((rc = dalloc(S, &a.partials, static_cast<size_t>(S->nblocks) * kMorseLanes + 64))) return rc;
+  if ((rc = dalloc(S, &a.partials_w, static_cast<size_t>(S->nblocks) * kMorseLanes + 64))) return rc;
   if ((rc = dalloc(S, &a.nhc, 64))) return rc;
   int32_t* cnt;
   if ((rc = dalloc(S, &cnt, round_up(N, 64)))) return rc;
