@@ -118,8 +118,10 @@ struct tdmd_pipe {
     for (auto* b : {&pos, &vel, &frc, &types, &ids, &masses, &ccount, &coff, &catoms, &acells,
                     &cwork, &counts, &offsets, &nbrs, &meta, &ovf, &swork, &scalar, &rwork})
       b->release();
-    for (auto e : events) (void)hipEventDestroy(e);
-    for (auto s : streams) (void)hipStreamDestroy(s);
+    for (auto e : events)
+      if (e) (void)hipEventDestroy(e);
+    for (auto s : streams)
+      if (s) (void)hipStreamDestroy(s);
     if (main) (void)hipStreamDestroy(main);
   }
 
