@@ -574,6 +574,11 @@ struct GridArgs {
   const int32_t* rank_of;  // linear cell id -> Morton rank
   const int32_t* around;   // [ncells][27] by rank: ranks of the 27 cells
                            // around the cell (periodic wrap), increasing
+  const uint8_t* around_img;  // the same entries' periodic images: 2 bits per
+                              // axis, 1 = +box length, 2 = -box length
+  int images;  // 1: every periodic axis has >= 3 cells no smaller than r_list,
+               // so a stencil entry's image is the minimum image of any pair
+               // within r_list that it holds
   int32_t* ccount;         // [ncells] by rank
   int32_t* cstart;         // [ncells] by rank
   int32_t* cursor;         // [ncells] by rank
@@ -779,9 +784,17 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
   const int32_t* __restrict__ around =
       g.around + 27 * static_cast<size_t>(
                           g.rank_of[cg.off + (c[2] * cg.n[1] + c[1]) * cg.n[0] + c[0]]);
+  const size_t row = static_cast<size_t>(around - g.around);
   for (int t = 0; t < 27; ++t) {
     const int r = around[t];
     const int start = g.cstart[r], end = start + g.ccount[r];
+    // With g.images the cell's image offset replaces the per-candidate
+    // minimum-image test: x_j - x_i then +-L, the same two roundings, so
+    // r2 is bit for bit that of the test for every pair within r_list.
+    const int img = g.images ? g.around_img[row + t] : 0;
+    const double sx = (img & 3) == 1 ? mp.L[0] : ((img & 3) == 2 ? -mp.L[0] : 0.0);
+    const double sy = ((img >> 2) & 3) == 1 ? mp.L[1] : (((img >> 2) & 3) == 2 ? -mp.L[1] : 0.0);
+    const double sz = ((img >> 4) & 3) == 1 ? mp.L[2] : (((img >> 4) & 3) == 2 ? -mp.L[2] : 0.0);
     // candidates in groups of four: the group's coordinates are loaded
     // together, then tested and appended in index order
     for (int j0 = start; j0 < end; j0 += 4) {
@@ -789,9 +802,16 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int j = min(j0 + q, end - 1);
-        const double ddx = min_image(a.px[j] - xi, mp.L[0], mp.hL[0], mp.periodic[0]);
-        const double ddy = min_image(a.py[j] - yi, mp.L[1], mp.hL[1], mp.periodic[1]);
-        const double ddz = min_image(a.pz[j] - zi, mp.L[2], mp.hL[2], mp.periodic[2]);
+        double ddx = a.px[j] - xi, ddy = a.py[j] - yi, ddz = a.pz[j] - zi;
+        if (g.images) {  // uniform
+          ddx = img & 3 ? ddx + sx : ddx;
+          ddy = (img >> 2) & 3 ? ddy + sy : ddy;
+          ddz = (img >> 4) & 3 ? ddz + sz : ddz;
+        } else {
+          ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
+          ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
+          ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
+        }
         r2q[q] = r2_unfused(ddx, ddy, ddz);
       }
 #pragma unroll

@@ -704,30 +704,53 @@ int sched_alloc(tdmd_sched* S) {
   // The list search's stencil: for each cell (row = its rank), the ranks of
   // the 27 cells around it with periodic wrap, in increasing rank, the order
   // in which the search visits them (atoms are sorted by cell rank, so rows
-  // come out in increasing atom index).
+  // come out in increasing atom index), and each entry's periodic image:
+  // +L where the stencil wrapped past the upper face, -L past the lower.
+  // When every periodic axis has at least 3 cells no smaller than r_list,
+  // a pair within r_list lies in exactly one stencil entry, whose image is
+  // its minimum image; the search then skips the per-candidate test.
   std::vector<int32_t> h_around(27 * static_cast<size_t>(g.ncells));
+  std::vector<uint8_t> h_img(27 * static_cast<size_t>(g.ncells));
+  bool images = true;
   for (int b = 0; b < nbox; ++b) {
     const int* n = S->multibox ? S->h_boxes[static_cast<size_t>(b)].n : g.n;
     const int off = S->multibox ? S->h_boxes[static_cast<size_t>(b)].cell_off : 0;
+    const MirrorParams& mp = S->multibox ? S->h_boxes[static_cast<size_t>(b)].mp : S->a.mp;
+    for (int d = 0; d < 3; ++d)
+      images = images && (!mp.periodic[d] || (n[d] >= 3 && mp.L[d] / n[d] >= S->r_list));
     for (int z = 0; z < n[2]; ++z)
       for (int y = 0; y < n[1]; ++y)
         for (int x = 0; x < n[0]; ++x) {
-          int32_t rk[27];
+          std::pair<int32_t, uint8_t> e[27];
           for (int t = 0; t < 27; ++t) {
-            const int nx = (x + t % 3 - 1 + n[0]) % n[0];
-            const int ny = (y + (t / 3) % 3 - 1 + n[1]) % n[1];
-            const int nz = (z + t / 9 - 1 + n[2]) % n[2];
-            rk[t] = h_rank[static_cast<size_t>(off + (nz * n[1] + ny) * n[0] + nx)];
+            const int c[3] = {x + t % 3 - 1, y + (t / 3) % 3 - 1, z + t / 9 - 1};
+            int w[3];
+            unsigned code = 0;
+            for (int d = 0; d < 3; ++d) {
+              w[d] = (c[d] + n[d]) % n[d];
+              if (mp.periodic[d] && c[d] >= n[d]) code |= 1u << (2 * d);  // +L
+              if (mp.periodic[d] && c[d] < 0) code |= 2u << (2 * d);      // -L
+            }
+            e[t] = {h_rank[static_cast<size_t>(off + (w[2] * n[1] + w[1]) * n[0] + w[0])],
+                    static_cast<uint8_t>(code)};
           }
-          std::sort(rk, rk + 27);
+          std::sort(e, e + 27);
           const int own = h_rank[static_cast<size_t>(off + (z * n[1] + y) * n[0] + x)];
-          std::copy(rk, rk + 27, h_around.begin() + 27 * static_cast<size_t>(own));
+          for (int t = 0; t < 27; ++t) {
+            h_around[27 * static_cast<size_t>(own) + t] = e[t].first;
+            h_img[27 * static_cast<size_t>(own) + t] = e[t].second;
+          }
         }
   }
   int32_t* around;
   if ((rc = dalloc(S, &around, h_around.size()))) return rc;
   TDMD_HIP_TRY(hipMemcpy(around, h_around.data(), h_around.size() * 4, hipMemcpyHostToDevice));
   g.around = around;
+  uint8_t* img;
+  if ((rc = dalloc(S, &img, h_img.size()))) return rc;
+  TDMD_HIP_TRY(hipMemcpy(img, h_img.data(), h_img.size(), hipMemcpyHostToDevice));
+  g.around_img = img;
+  g.images = images ? 1 : 0;
   if (S->multibox) {
     BoxGeo* boxes;
     int32_t* rep;

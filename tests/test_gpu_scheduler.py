@@ -173,6 +173,27 @@ def test_displacement_rebuild_policy():
     assert force_rel_err(f, rf) < 1e-8
 
 
+@pytest.mark.parametrize("pot", ["johnson", "morse"])
+def test_large_box_trajectory_matches_oracle(pot):
+    # 7^3 cells (25.3 A): at least three cells no smaller than r_list per
+    # axis, so the list search takes each stencil cell's periodic image
+    # instead of a per-candidate minimum-image test; the rows (and so the
+    # FP64 trajectory) must be those of the oracle's list
+    s = fcc(7)
+    say("large box", pot, s.natoms)
+    sched, pos, vel, f = gpu_run(s, pot, 40, precision="fp64", rebuild_every=5)
+    rpos, rvel, rf, th, reb = O.run(s, eam=None if pot == "morse" else O.Eam(tables=potential(pot)),
+                                    morse=MORSE_CU if pot == "morse" else None, nsteps=40,
+                                    thermo_every=40, rebuild_mode=1, rebuild_every=5)
+    L = s.hi - s.lo
+    dpos = pos - rpos
+    dpos -= np.round(dpos / L) * L
+    say("  max dpos", np.abs(dpos).max())
+    assert np.abs(dpos).max() < 1e-10
+    assert force_rel_err(f, rf) < 1e-9
+    assert sched.stats().rebuilds == 1 + 7
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("pot,chunks", [("johnson", None), ("johnson", [7, 31, 60]),
                                         ("morse", None), ("mishin", [25, 60])])
