@@ -709,6 +709,25 @@ __device__ __forceinline__ void order_cell(const GridArgs& g, int r) {
   }
 }
 
+// The same order by one wavefront: lane l takes the cell's l-th atom and
+// counts the atoms with a smaller upload index (indices are unique, so the
+// counts are the sorted positions); all lanes read before any writes.
+// Cells of more than 64 atoms fall back to one lane's insertion sort.
+__device__ __forceinline__ void order_cell_wave(const GridArgs& g, int r) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  int32_t* seg = g.sidx + g.cstart[r];
+  const int n = g.ccount[r];
+  if (n > 64) {
+    if (lane == 0) order_cell(g, r);
+    return;
+  }
+  const int32_t v = lane < n ? seg[lane] : 0;
+  const int32_t kv = lane < n ? g.orig[v] : 0x7fffffff;
+  int pos = 0;
+  for (int m = 0; m < n; ++m) pos += __shfl(kv, m, 64) < kv ? 1 : 0;
+  if (lane < n) seg[pos] = v;
+}
+
 template <class R>
 __device__ __forceinline__ void permute_atom(const StepArgs& a, const GridArgs& g,
                                              const PermArgs& t, int s) {
@@ -1189,11 +1208,10 @@ k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double nea
   for (int i = tid; i < N; i += nthr) scatter_atom(g, i);
   if (!grid_barrier(bar, a.flags)) return;
   {
-    // one cell per thread, cells dealt to the workgroups wave by wave so the
-    // few cells (far fewer than threads) spread over every CU
+    // one cell per wavefront, cells dealt to the workgroups wave by wave
     const int nw = gridDim.x * (kRebuildBlock / 64);
     const int wv = (threadIdx.x >> 6) * gridDim.x + blockIdx.x;
-    for (int r = wv * 64 + (threadIdx.x & 63); r < g.ncells; r += nw * 64) order_cell(g, r);
+    for (int r = wv; r < g.ncells; r += nw) order_cell_wave(g, r);
   }
   if (!grid_barrier(bar, a.flags)) return;
   for (int s = tid; s < N; s += nthr) permute_atom<R>(a, g, t, s);
