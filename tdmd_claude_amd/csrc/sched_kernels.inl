@@ -1036,8 +1036,20 @@ __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, Halo
       const int i = b * kHaloBlock + t;
       if (i >= a.natoms) break;
       halo_insert(L.key, i, &L.nd, vovf);  // row padding names i itself
-      const int n = a.cnt[i];
-      for (int k = 0; k < n && !*vovf; ++k) halo_insert(L.key, a.nbr[ell(i, k, cap)], &L.nd, vovf);
+      // the row in quads (four adjacent slots per load), the next quad
+      // loaded while this one is inserted; slots past the count in the last
+      // quad are padding (i, already present)
+      const int nq = (a.cnt[i] + 3) >> 2;
+      const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
+      int4 cur = nq > 0 ? q4[ell(i, 0, cap) >> 2] : make_int4(i, i, i, i);
+      for (int q = 0; q < nq && !*vovf; ++q) {
+        const int4 nxt = q + 1 < nq ? q4[ell(i, 4 * (q + 1), cap) >> 2] : cur;
+        halo_insert(L.key, cur.x, &L.nd, vovf);
+        if (!*vovf) halo_insert(L.key, cur.y, &L.nd, vovf);
+        if (!*vovf) halo_insert(L.key, cur.z, &L.nd, vovf);
+        if (!*vovf) halo_insert(L.key, cur.w, &L.nd, vovf);
+        cur = nxt;
+      }
     }
     __syncthreads();
     const int H = L.nd;
@@ -1092,10 +1104,20 @@ __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, Halo
     {
       const int i = b * kHaloBlock + tid;
       if (i < a.natoms) {
-        const int m = (a.cnt[i] + 7) & ~7;  // the padded row
-        for (int k = 0; k < m; ++k) {
-          const int j = a.nbr[ell(i, k, cap)];
-          nbr16[ell16(i, k, cap)] = static_cast<uint16_t>(L.srt[halo_find(L.key, j)]);
+        // the padded row in quads; a quad's four 16-bit positions are
+        // adjacent in the 16-bit layout (eight slots per 16 bytes)
+        const int mq = ((a.cnt[i] + 7) & ~7) >> 2;
+        const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
+        int4 cur = mq > 0 ? q4[ell(i, 0, cap) >> 2] : make_int4(i, i, i, i);
+        for (int q = 0; q < mq; ++q) {
+          const int4 nxt = q + 1 < mq ? q4[ell(i, 4 * (q + 1), cap) >> 2] : cur;
+          const uint32_t p0 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.x)]);
+          const uint32_t p1 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.y)]);
+          const uint32_t p2 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.z)]);
+          const uint32_t p3 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.w)]);
+          *reinterpret_cast<uint2*>(nbr16 + ell16(i, 4 * q, cap)) =
+              make_uint2(p0 | (p1 << 16), p2 | (p3 << 16));
+          cur = nxt;
         }
       }
     }
