@@ -804,50 +804,73 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
       g.around + 27 * static_cast<size_t>(
                           g.rank_of[cg.off + (c[2] * cg.n[1] + c[1]) * cg.n[0] + c[0]]);
   const size_t row = static_cast<size_t>(around - g.around);
-  for (int t = 0; t < 27; ++t) {
-    const int r = around[t];
-    const int start = g.cstart[r], end = start + g.ccount[r];
-    // With g.images the cell's image offset replaces the per-candidate
-    // minimum-image test: x_j - x_i then +-L, the same two roundings, so
-    // r2 is bit for bit that of the test for every pair within r_list.
-    const int img = g.images ? g.around_img[row + t] : 0;
-    const double sx = (img & 3) == 1 ? mp.L[0] : ((img & 3) == 2 ? -mp.L[0] : 0.0);
-    const double sy = ((img >> 2) & 3) == 1 ? mp.L[1] : (((img >> 2) & 3) == 2 ? -mp.L[1] : 0.0);
-    const double sz = ((img >> 4) & 3) == 1 ? mp.L[2] : (((img >> 4) & 3) == 2 ? -mp.L[2] : 0.0);
-    // candidates in groups of four: the group's coordinates are loaded
-    // together, then tested and appended in index order
-    for (int j0 = start; j0 < end; j0 += 4) {
-      double r2q[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = min(j0 + q, end - 1);
-        double ddx = a.px[j] - xi, ddy = a.py[j] - yi, ddz = a.pz[j] - zi;
-        if (g.images) {  // uniform
-          ddx = img & 3 ? ddx + sx : ddx;
-          ddy = (img >> 2) & 3 ? ddy + sy : ddy;
-          ddz = (img >> 4) & 3 ? ddz + sz : ddz;
-        } else {
-          ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
-          ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
-          ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
-        }
-        r2q[q] = r2_unfused(ddx, ddy, ddz);
+  // One copy of the search per image mode, so the loop body has no branch
+  // between a group's coordinate loads and they issue together.
+  auto search = [&](auto images) {
+    constexpr bool kImg = decltype(images)::value;
+    for (int t = 0; t < 27; ++t) {
+      const int r = around[t];
+      const int start = g.cstart[r], end = start + g.ccount[r];
+      // kImg: the cell's image offset replaces the per-candidate minimum
+      // image test: x_j - x_i then +-L, the same two roundings, so r2 is
+      // bit for bit that of the test for every pair within r_list
+      double sx = 0.0, sy = 0.0, sz = 0.0;
+      bool ix = false, iy = false, iz = false;
+      if constexpr (kImg) {
+        const int img = g.around_img[row + t];
+        ix = (img & 3) != 0;
+        iy = ((img >> 2) & 3) != 0;
+        iz = ((img >> 4) & 3) != 0;
+        sx = (img & 3) == 1 ? mp.L[0] : -mp.L[0];
+        sy = ((img >> 2) & 3) == 1 ? mp.L[1] : -mp.L[1];
+        sz = ((img >> 4) & 3) == 1 ? mp.L[2] : -mp.L[2];
       }
+      // candidates in groups of four: the group's coordinates are loaded
+      // together, then tested and appended in index order
+      for (int j0 = start; j0 < end; j0 += 4) {
+        double px[4], py[4], pz[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = j0 + q;
-        if (j < end && j != i && r2q[q] < r_list_sq) {
-          if (!kCount && nn + nf < cap) {
-            if (r2q[q] < near_sq)
-              nbr[ell(i, nn++, cap)] = j;
-            else
-              nbr[ell(i, cap - 1 - nf++, cap)] = j;
+        for (int q = 0; q < 4; ++q) {
+          const int j = min(j0 + q, end - 1);
+          px[q] = a.px[j];
+          py[q] = a.py[j];
+          pz[q] = a.pz[j];
+        }
+        double r2q[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double ddx = px[q] - xi, ddy = py[q] - yi, ddz = pz[q] - zi;
+          if constexpr (kImg) {
+            ddx = ix ? ddx + sx : ddx;
+            ddy = iy ? ddy + sy : ddy;
+            ddz = iz ? ddz + sz : ddz;
+          } else {
+            ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
+            ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
+            ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
           }
-          ++count;
+          r2q[q] = r2_unfused(ddx, ddy, ddz);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = j0 + q;
+          if (j < end && j != i && r2q[q] < r_list_sq) {
+            if (!kCount && nn + nf < cap) {
+              if (r2q[q] < near_sq)
+                nbr[ell(i, nn++, cap)] = j;
+              else
+                nbr[ell(i, cap - 1 - nf++, cap)] = j;
+            }
+            ++count;
+          }
         }
       }
     }
-  }
+  };
+  if (g.images)
+    search(std::true_type{});
+  else
+    search(std::false_type{});
   if (!kCount) {
     // Shell entries sit at [cap - nf, cap); the gap [nn, cap - nf) has
     // room for the m that must move (the rest already lie in [nn, nn+nf)).
