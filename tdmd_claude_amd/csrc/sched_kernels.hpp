@@ -161,7 +161,8 @@ struct BoxGeo {
   int n[3];
   double inv_cs[3];
   int cell_off;
-  int pad;
+  int images;  // 1: the list search may take stencil images (>= 3 cells of
+               // >= r_list on every periodic axis)
 };
 
 // ELL index of the k-th neighbour of atom i. A wavefront's 64 rows are

@@ -578,7 +578,7 @@ struct GridArgs {
                               // axis, 1 = +box length, 2 = -box length
   int images;  // 1: every periodic axis has >= 3 cells no smaller than r_list,
                // so a stencil entry's image is the minimum image of any pair
-               // within r_list that it holds
+               // within r_list that it holds (batches: per box, BoxGeo)
   int32_t* ccount;         // [ncells] by rank
   int32_t* cstart;         // [ncells] by rank
   int32_t* cursor;         // [ncells] by rank
@@ -867,7 +867,12 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
       }
     }
   };
-  if (g.images)
+  bool images;
+  if constexpr (Mir::kMB)
+    images = a.boxes[a.rep[i]].images != 0;
+  else
+    images = g.images != 0;
+  if (images)
     search(std::true_type{});
   else
     search(std::false_type{});

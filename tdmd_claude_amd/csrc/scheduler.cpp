@@ -716,8 +716,11 @@ int sched_alloc(tdmd_sched* S) {
     const int* n = S->multibox ? S->h_boxes[static_cast<size_t>(b)].n : g.n;
     const int off = S->multibox ? S->h_boxes[static_cast<size_t>(b)].cell_off : 0;
     const MirrorParams& mp = S->multibox ? S->h_boxes[static_cast<size_t>(b)].mp : S->a.mp;
+    bool box_images = true;
     for (int d = 0; d < 3; ++d)
-      images = images && (!mp.periodic[d] || (n[d] >= 3 && mp.L[d] / n[d] >= S->r_list));
+      box_images = box_images && (!mp.periodic[d] || (n[d] >= 3 && mp.L[d] / n[d] >= S->r_list));
+    if (S->multibox) S->h_boxes[static_cast<size_t>(b)].images = box_images ? 1 : 0;
+    images = images && box_images;
     for (int z = 0; z < n[2]; ++z)
       for (int y = 0; y < n[1]; ++y)
         for (int x = 0; x < n[0]; ++x) {
@@ -750,7 +753,7 @@ int sched_alloc(tdmd_sched* S) {
   if ((rc = dalloc(S, &img, h_img.size()))) return rc;
   TDMD_HIP_TRY(hipMemcpy(img, h_img.data(), h_img.size(), hipMemcpyHostToDevice));
   g.around_img = img;
-  g.images = images ? 1 : 0;
+  g.images = images ? 1 : 0;  // batches: decided per box (BoxGeo::images)
   if (S->multibox) {
     BoxGeo* boxes;
     int32_t* rep;
