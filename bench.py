@@ -39,6 +39,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+THERMAL_STEPS = 1100  # NVT steps before the pass timing: the default warmup + timed steps
+
+
 def make_system(cells: int) -> I.System:
     s = I.fcc_lattice(cells, cells, cells, 3.615)
     I.init_velocities(s, 300.0, 7)
@@ -323,14 +326,21 @@ def main() -> int:
                                                   "mirror_fp64": args.mirror_fp64}))
         wall, dev = timed_run(sched, args.warmup, args.steps)
         log(f"profile run: {args.steps} steps in {wall:.3f} s")
+        # on to the state the benchmark's pass timing sees (1100 NVT steps)
+        sched.run_until(max(THERMAL_STEPS, args.warmup + args.steps))
+        pos_t, vel_t, _, _, _ = sched.download()
+        sched.close()
+        state = system.copy()
+        state.pos, state.vel = pos_t, vel_t
         nve = FastPipelineScheduler.for_system(
-            system, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
-        ta, tb = nve.time_passes(20)
-        log(f"passes: density {ta:.4f} ms, force {tb:.4f} ms")
+            state, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
+        ta, tb = nve.time_passes(30)
+        log(f"passes (thermalised state): density {ta:.4f} ms, force {tb:.4f} ms")
+        nve.close()
         # a list rebuild every 5 steps: the fused rebuild's own cost in a trace
         reb = FastPipelineScheduler.for_system(
-            system, tables, FastPipelineConfig(precision="mixed", rebuild_policy="fixed",
-                                               rebuild_every=5, mirror_fp64=args.mirror_fp64))
+            state, tables, FastPipelineConfig(precision="mixed", rebuild_policy="fixed",
+                                              rebuild_every=5, mirror_fp64=args.mirror_fp64))
         wall, _ = timed_run(reb, 0, 20)
         log(f"rebuild every 5 steps: 20 steps in {wall:.4f} s")
         return 0
