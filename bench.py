@@ -128,6 +128,46 @@ def bench_morse_c2(steps: int) -> dict:
     return out
 
 
+# The reference's published mixed-precision timesteps per second for its
+# tiny / small / medium Cu systems (RTX 5080, CUDA 12.6), the best figure
+# published for each: Morse with the relative-coordinate trick
+# (docs/06-decisions/0007-precision-contract.md:369-371), EAM from the
+# LAMMPS A/B session (docs/05-benchmarks/lammps-ab-results.md:38-52).
+REF_SIZES = (("tiny", 4), ("small", 10), ("medium", 20))
+REF_TS = {"morse": (14814, 9139, 6927), "eam": (4802, 3288, 2645)}
+
+
+def bench_reference_sizes(steps: int) -> dict:
+    """The reference's own benchmark systems: FCC Cu (a = 3.615 A) of 4^3,
+    10^3 and 20^3 cells (256, 4000, 32000 atoms), Morse (D 0.3429 eV,
+    alpha 1.3588 1/A, r0 2.866 A, rc 6.0 A) and Mishin Cu EAM (its setfl:
+    10001-point tables, rc 5.50679 A), mixed precision, NVE, dt 1 fs, skin
+    1.0 A, list rebuilt every 10 steps, 300 K velocities (seed 42). Timesteps
+    per second beside the reference's published figures for the same
+    systems."""
+    eam = I.read_setfl(os.path.join(ROOT, "tests", "golden", "Cu_mishin1.eam.alloy"))
+    morse = (0.3429, 1.3588, 2.866, 6.0)
+    out = {"workload": "FCC Cu 256 / 4000 / 32000 atoms, Morse and Mishin Cu EAM, mixed, NVE, "
+                       "dt 1 fs, skin 1 A, rebuild every 10 steps",
+           "steps": steps, "reference_hardware": "RTX 5080, CUDA 12.6"}
+    for pot, params in (("morse", morse), ("eam", eam)):
+        rows = {}
+        for (name, cells), ref_ts in zip(REF_SIZES, REF_TS[pot]):
+            s = I.fcc_lattice(cells, cells, cells, 3.615)
+            I.init_velocities(s, 300.0, 42)
+            cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision="mixed",
+                                     rebuild_policy="fixed", rebuild_every=10)
+            sched = FastPipelineScheduler.for_system(s, params, cfg)
+            wall, _ = timed_run(sched, 100, steps)
+            ts = steps / wall
+            rows[name] = {"natoms": s.natoms, "ts_per_s": round(ts, 1),
+                          "matom_steps_per_s": round(s.natoms * ts / 1e6, 2),
+                          "reference_ts_per_s": ref_ts, "over_reference": round(ts / ref_ts, 2)}
+            sched.close()
+        out[pot] = rows
+    return out
+
+
 def bench_replicas(steps: int) -> dict:
     """64 replicas of random equiatomic Ni/Co/Cr FCC alloys (a = 3.57 A, sizes
     log-uniform in [500, 32000], T ~ U(300, 1200) K, master seed 2024),
@@ -279,6 +319,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=10000,
                     help="Morse 32k-atom NVE steps per precision (0: skip)")
+    ap.add_argument("--ref-steps", type=int, default=5000,
+                    help="steps per system of the reference-size legs (0: skip)")
     ap.add_argument("--replica-steps", type=int, default=1000,
                     help="64-replica alloy steps per precision (0: skip)")
     ap.add_argument("--fm-iters", type=int, default=200,
@@ -399,6 +441,12 @@ def main() -> int:
         extras["morse_32k"] = bench_morse_c2(args.c2_steps)
         log(f"Morse 32k: mixed {extras['morse_32k']['mixed']['value']} / fp64 "
             f"{extras['morse_32k']['fp64']['value']} Matom-steps/s")
+    if args.ref_steps > 0:
+        extras["reference_sizes"] = bench_reference_sizes(args.ref_steps)
+        for pot in ("morse", "eam"):
+            log(f"reference sizes, {pot}: " + ", ".join(
+                f"{k} {v['ts_per_s']} ts/s ({v['over_reference']}x)"
+                for k, v in extras["reference_sizes"][pot].items()))
     if args.replica_steps > 0:
         extras["replicas_64"] = bench_replicas(args.replica_steps)
         log(f"64 replicas ({extras['replicas_64']['natoms']} atoms): mixed "

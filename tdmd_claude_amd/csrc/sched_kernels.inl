@@ -579,6 +579,7 @@ struct GridArgs {
   int images;  // 1: every periodic axis has >= 3 cells no smaller than r_list,
                // so a stencil entry's image is the minimum image of any pair
                // within r_list that it holds (batches: per box, BoxGeo)
+  int nl_split;  // 1: the fused rebuild searches each atom's cells with 32 lanes
   int32_t* ccount;         // [ncells] by rank
   int32_t* cstart;         // [ncells] by rank
   int32_t* cursor;         // [ncells] by rank
@@ -892,6 +893,106 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
   return count;
 }
 
+// nlist_atom with 32 lanes per atom (two atoms per wavefront), for systems
+// too small to fill the chip one lane per atom: each cell's candidates are
+// tested 32 at a time, and a hit's place in the row is its rank among the
+// hits in candidate order (ballot, then a prefix count). A first sweep
+// counts the near and shell entries kept (hit k is kept when k < cap); the
+// second writes the row exactly as nlist_atom leaves it: near entries from
+// the front, shell entry q at cap - 1 - q and, for q < m, also at nn + q
+// (the shell block moved behind the near block), then the padding. Distance
+// arithmetic is nlist_atom's, so the row is bit for bit the same. Every lane
+// of the wave runs both sweeps (the ballots need them); `act` is false for
+// the half with no atom, whose i is any valid atom.
+template <class Mir>
+__device__ __forceinline__ int nlist_atom_group(const StepArgs& a, const GridArgs& g, int i,
+                                                bool act, double r_list_sq, double near_sq,
+                                                int32_t* nbr, int32_t* cnt) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int l = lane & 31, sh = lane & 32;
+  const uint32_t lt = (1u << l) - 1u;
+  const int cap = a.cap;
+  const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
+  const MirrorParams& mp = mp_of<Mir>(a, i);
+  const CellGrid cg = grid_of<Mir>(a, g, i);
+  int c[3];
+  cell_coords(cg, xi, yi, zi, c);
+  const size_t row =
+      27 * static_cast<size_t>(g.rank_of[cg.off + (c[2] * cg.n[1] + c[1]) * cg.n[0] + c[0]]);
+  bool images;
+  if constexpr (Mir::kMB)
+    images = a.boxes[a.rep[i]].images != 0;
+  else
+    images = g.images != 0;
+  int count = 0, nn = 0, nf = 0;
+  int nn_all = 0, m = 0;  // second sweep: final near count, shell entries moved
+  auto sweep = [&](auto write) {
+    constexpr bool kWrite = decltype(write)::value;
+    count = nn = nf = 0;
+    for (int t = 0; t < 27; ++t) {
+      const int r = g.around[row + t];
+      const int start = act ? g.cstart[r] : 0;
+      const int end = act ? start + g.ccount[r] : 0;
+      const int img = g.around_img[row + t];
+      const bool ix = images && (img & 3) != 0;
+      const bool iy = images && ((img >> 2) & 3) != 0;
+      const bool iz = images && ((img >> 4) & 3) != 0;
+      const double sx = (img & 3) == 1 ? mp.L[0] : -mp.L[0];
+      const double sy = ((img >> 2) & 3) == 1 ? mp.L[1] : -mp.L[1];
+      const double sz = ((img >> 4) & 3) == 1 ? mp.L[2] : -mp.L[2];
+      for (int j0 = start; __any(j0 < end); j0 += 32) {
+        const int j = j0 + l;
+        const bool valid = j < end;
+        const int jj = valid ? j : i;
+        double ddx = a.px[jj] - xi, ddy = a.py[jj] - yi, ddz = a.pz[jj] - zi;
+        if (images) {
+          ddx = ix ? ddx + sx : ddx;
+          ddy = iy ? ddy + sy : ddy;
+          ddz = iz ? ddz + sz : ddz;
+        } else {
+          ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
+          ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
+          ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
+        }
+        const double r2 = r2_unfused(ddx, ddy, ddz);
+        const bool hit = valid && j != i && r2 < r_list_sq;
+        const uint32_t hg = static_cast<uint32_t>(__ballot(hit) >> sh);
+        const bool kept = hit && count + __popc(hg & lt) < cap;
+        const bool near = kept && r2 < near_sq;
+        const uint32_t kg = static_cast<uint32_t>(__ballot(kept) >> sh);
+        const uint32_t ng = static_cast<uint32_t>(__ballot(near) >> sh);
+        if (kWrite && kept) {
+          if (near) {
+            nbr[ell(i, nn + __popc(ng & lt), cap)] = j;
+          } else {
+            const int q = nf + __popc(kg & ~ng & lt);
+            nbr[ell(i, cap - 1 - q, cap)] = j;
+            if (q < m) nbr[ell(i, nn_all + q, cap)] = j;
+          }
+        }
+        count += __popc(hg);
+        nn += __popc(ng);
+        nf += __popc(kg & ~ng);
+      }
+    }
+  };
+  sweep(std::false_type{});
+  nn_all = nn;
+  const int gap = cap - nn - nf;
+  m = nf < gap ? nf : gap;
+  sweep(std::true_type{});
+  if (act) {
+    const int n = nn + nf;  // == min(count, cap)
+    if (l == 0) {
+      cnt[i] = n;
+      a.cntn[i] = nn;
+    }
+    const int k = n + l;
+    if (k < ((n + 7) & ~7)) nbr[ell(i, k, cap)] = i;
+  }
+  return act ? count : 0;
+}
+
 // Largest row count of the wave: recorded for sizing (kCount) or flagged as
 // an overflow of the capacity.
 template <bool kCount>
@@ -1193,9 +1294,16 @@ __global__ void __launch_bounds__(kHaloBlock) k_halo(StepArgs a, int nblocks, in
 // with a growing sleep.
 // ---------------------------------------------------------------------------
 
+// Arrival is two-level: the workgroups of XCD group x (blockIdx % 8 == x)
+// count at sub[32 x], and the last of each group counts at `count`; the
+// last of those bumps `gen`. Arrivals on one address are serialised, so 512
+// workgroups on a single counter took ~30 us a barrier; eight counters of
+// at most 64 arrivals each, on separate 128-byte lines, take a fraction.
 struct GridBarrier {
-  unsigned int count;
-  unsigned int gen;
+  unsigned int count;  // groups arrived
+  unsigned int gen;    // generation: bumped when every group has arrived
+  unsigned int pad[30];
+  unsigned int sub[8 * 32];  // workgroups arrived in group x: sub[32 x]
 };
 
 constexpr long kBarrierSpins = 1L << 22;
@@ -1207,9 +1315,20 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, int32_t* flags) {
     ok = 1;
     const unsigned int gen =
         __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int x = blockIdx.x & 7u;
+    const unsigned int nx = (gridDim.x - x + 7u) >> 3;  // workgroups in group x
+    const unsigned int ngroups = gridDim.x < 8u ? gridDim.x : 8u;
+    unsigned int* sub = &b->sub[32u * x];
     const unsigned int arrived =
-        __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == gridDim.x - 1) {
+        __hip_atomic_fetch_add(sub, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    bool last = false;
+    if (arrived == nx - 1u) {  // last of its group: arrive at the top
+      __hip_atomic_store(sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned int top =
+          __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      last = top == ngroups - 1u;
+    }
+    if (last) {
       __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&b->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -1269,9 +1388,21 @@ k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double nea
   for (int s = tid; s < N; s += nthr) copyback_atom<R, Mir>(a, g, t, s);
   if (!grid_barrier(bar, a.flags)) return;
   int count = 0;
-  for (int i = tid; i < N; i += nthr) {
-    const int c = nlist_atom<false, Mir>(a, g, i, r_list_sq, near_sq, nbr, cnt);
-    count = c > count ? c : count;
+  if (g.nl_split) {
+    // two atoms per wavefront, 32 lanes each; the loop bound is uniform per wave
+    const int gw = tid >> 6, nw = nthr >> 6;
+    for (int base = 2 * gw; base < N; base += 2 * nw) {
+      const int i = base + static_cast<int>((threadIdx.x >> 5) & 1);
+      const bool act = i < N;
+      const int c = nlist_atom_group<Mir>(a, g, act ? i : base, act, r_list_sq, near_sq, nbr,
+                                          cnt);
+      count = c > count ? c : count;
+    }
+  } else {
+    for (int i = tid; i < N; i += nthr) {
+      const int c = nlist_atom<false, Mir>(a, g, i, r_list_sq, near_sq, nbr, cnt);
+      count = c > count ? c : count;
+    }
   }
   report_count<false>(a, count, nullptr);
   if constexpr (Mir::kFixed) {

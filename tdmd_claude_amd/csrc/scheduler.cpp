@@ -44,6 +44,8 @@ using namespace sched;
 // workgroups on 256 CUs, each lane walking ~80 slots in turn.
 constexpr int kMorseLanes = 8;
 constexpr int kMorseSplitMax = 131072;
+// The fused rebuild's list search takes 32 lanes per atom up to this size.
+constexpr int kNlistSplitMax = 65536;
 
 // Operation codes of a step sequence (also the graph-cache key alphabet).
 enum Op : char {
@@ -261,10 +263,16 @@ struct Engine final : EngineBase {
       TDMD_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
       TDMD_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &per_cu, reinterpret_cast<const void*>(&k_rebuild_fused<R, Mir>), kRebuildBlock, 0));
-      // at most two workgroups per CU: a margin under the occupancy limit
-      const int need = std::max((S->natoms + kRebuildBlock - 1) / kRebuildBlock, 1);
+      // at most two workgroups per CU: a margin under the occupancy limit.
+      // Small systems search each atom's cells with 32 lanes (one lane per
+      // atom leaves the search a chain of dependent loads on a few CUs).
+      const bool split = S->natoms <= kNlistSplitMax;
+      const int64_t lanes = static_cast<int64_t>(S->natoms) * (split ? 32 : 1);
+      const int need = static_cast<int>(
+          std::max<int64_t>((lanes + kRebuildBlock - 1) / kRebuildBlock, 1));
       S->nb_rebuild = std::max(1, std::min(std::min(per_cu, 2) * ncu, need));
       if (per_cu < 1) S->nb_rebuild = 0;  // cannot guarantee residency: separate kernels
+      S->g.nl_split = split && S->nb_rebuild > 0 ? 1 : 0;
     }
     if (S->kind != 1 || S->tables->ntypes != 1) return TDMD_OK;
     // rho_0 and phi_00 must share the r grid (setfl: one Nr, dr).
