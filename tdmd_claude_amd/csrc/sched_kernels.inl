@@ -894,16 +894,17 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
 }
 
 // nlist_atom with 32 lanes per atom (two atoms per wavefront), for systems
-// too small to fill the chip one lane per atom: each cell's candidates are
-// tested 32 at a time, and a hit's place in the row is its rank among the
-// hits in candidate order (ballot, then a prefix count). A first sweep
-// counts the near and shell entries kept (hit k is kept when k < cap); the
-// second writes the row exactly as nlist_atom leaves it: near entries from
-// the front, shell entry q at cap - 1 - q and, for q < m, also at nn + q
-// (the shell block moved behind the near block), then the padding. Distance
-// arithmetic is nlist_atom's, so the row is bit for bit the same. Every lane
-// of the wave runs both sweeps (the ballots need them); `act` is false for
-// the half with no atom, whose i is any valid atom.
+// too small to fill the chip one lane per atom. Lane t < 27 of a group loads
+// stencil entry t's cell range and image, shared by shuffles; each cell's
+// candidates are then tested 32 at a time, and a hit's place in the row is
+// its rank among the hits in candidate order (ballot, then a prefix count):
+// hit k is kept when k < cap, near entries go to the front and shell entry
+// q to cap - 1 - q, as in nlist_atom. After a workgroup-scope fence (this
+// wave's row stores complete before it reads them back) the shell block is
+// moved behind the near block and the row padded, as nlist_atom does.
+// Distance arithmetic is nlist_atom's, so the row is bit for bit the same.
+// Every lane of the wave runs the loop (the ballots and shuffles need
+// them); `act` is false for a half with no atom, whose i is any valid atom.
 template <class Mir>
 __device__ __forceinline__ int nlist_atom_group(const StepArgs& a, const GridArgs& g, int i,
                                                 bool act, double r_list_sq, double near_sq,
@@ -924,64 +925,65 @@ __device__ __forceinline__ int nlist_atom_group(const StepArgs& a, const GridArg
     images = a.boxes[a.rep[i]].images != 0;
   else
     images = g.images != 0;
+  int my_start = 0, my_end = 0, my_img = 0;
+  if (act && l < 27) {
+    const int r = g.around[row + l];
+    my_start = g.cstart[r];
+    my_end = my_start + g.ccount[r];
+    my_img = g.around_img[row + l];
+  }
   int count = 0, nn = 0, nf = 0;
-  int nn_all = 0, m = 0;  // second sweep: final near count, shell entries moved
-  auto sweep = [&](auto write) {
-    constexpr bool kWrite = decltype(write)::value;
-    count = nn = nf = 0;
-    for (int t = 0; t < 27; ++t) {
-      const int r = g.around[row + t];
-      const int start = act ? g.cstart[r] : 0;
-      const int end = act ? start + g.ccount[r] : 0;
-      const int img = g.around_img[row + t];
-      const bool ix = images && (img & 3) != 0;
-      const bool iy = images && ((img >> 2) & 3) != 0;
-      const bool iz = images && ((img >> 4) & 3) != 0;
-      const double sx = (img & 3) == 1 ? mp.L[0] : -mp.L[0];
-      const double sy = ((img >> 2) & 3) == 1 ? mp.L[1] : -mp.L[1];
-      const double sz = ((img >> 4) & 3) == 1 ? mp.L[2] : -mp.L[2];
-      for (int j0 = start; __any(j0 < end); j0 += 32) {
-        const int j = j0 + l;
-        const bool valid = j < end;
-        const int jj = valid ? j : i;
-        double ddx = a.px[jj] - xi, ddy = a.py[jj] - yi, ddz = a.pz[jj] - zi;
-        if (images) {
-          ddx = ix ? ddx + sx : ddx;
-          ddy = iy ? ddy + sy : ddy;
-          ddz = iz ? ddz + sz : ddz;
-        } else {
-          ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
-          ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
-          ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
-        }
-        const double r2 = r2_unfused(ddx, ddy, ddz);
-        const bool hit = valid && j != i && r2 < r_list_sq;
-        const uint32_t hg = static_cast<uint32_t>(__ballot(hit) >> sh);
-        const bool kept = hit && count + __popc(hg & lt) < cap;
-        const bool near = kept && r2 < near_sq;
-        const uint32_t kg = static_cast<uint32_t>(__ballot(kept) >> sh);
-        const uint32_t ng = static_cast<uint32_t>(__ballot(near) >> sh);
-        if (kWrite && kept) {
-          if (near) {
-            nbr[ell(i, nn + __popc(ng & lt), cap)] = j;
-          } else {
-            const int q = nf + __popc(kg & ~ng & lt);
-            nbr[ell(i, cap - 1 - q, cap)] = j;
-            if (q < m) nbr[ell(i, nn_all + q, cap)] = j;
-          }
-        }
-        count += __popc(hg);
-        nn += __popc(ng);
-        nf += __popc(kg & ~ng);
+  for (int t = 0; t < 27; ++t) {
+    const int start = __shfl(my_start, sh + t);
+    const int end = __shfl(my_end, sh + t);
+    const int img = __shfl(my_img, sh + t);
+    const bool ix = images && (img & 3) != 0;
+    const bool iy = images && ((img >> 2) & 3) != 0;
+    const bool iz = images && ((img >> 4) & 3) != 0;
+    const double sx = (img & 3) == 1 ? mp.L[0] : -mp.L[0];
+    const double sy = ((img >> 2) & 3) == 1 ? mp.L[1] : -mp.L[1];
+    const double sz = ((img >> 4) & 3) == 1 ? mp.L[2] : -mp.L[2];
+    for (int j0 = start; __any(j0 < end); j0 += 32) {
+      const int j = j0 + l;
+      const bool valid = j < end;
+      const int jj = valid ? j : i;
+      double ddx = a.px[jj] - xi, ddy = a.py[jj] - yi, ddz = a.pz[jj] - zi;
+      if (images) {
+        ddx = ix ? ddx + sx : ddx;
+        ddy = iy ? ddy + sy : ddy;
+        ddz = iz ? ddz + sz : ddz;
+      } else {
+        ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
+        ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
+        ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
       }
+      const double r2 = r2_unfused(ddx, ddy, ddz);
+      const bool hit = valid && j != i && r2 < r_list_sq;
+      const uint32_t hg = static_cast<uint32_t>(__ballot(hit) >> sh);
+      const bool kept = hit && count + __popc(hg & lt) < cap;
+      const bool near = kept && r2 < near_sq;
+      const uint32_t kg = static_cast<uint32_t>(__ballot(kept) >> sh);
+      const uint32_t ng = static_cast<uint32_t>(__ballot(near) >> sh);
+      if (kept) {
+        if (near)
+          nbr[ell(i, nn + __popc(ng & lt), cap)] = j;
+        else
+          nbr[ell(i, cap - 1 - (nf + __popc(kg & ~ng & lt)), cap)] = j;
+      }
+      count += __popc(hg);
+      nn += __popc(ng);
+      nf += __popc(kg & ~ng);
     }
-  };
-  sweep(std::false_type{});
-  nn_all = nn;
-  const int gap = cap - nn - nf;
-  m = nf < gap ? nf : gap;
-  sweep(std::true_type{});
+  }
+  // the row stores above are complete before the shell entries are read back
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   if (act) {
+    // Shell entries sit at [cap - nf, cap); the gap [nn, cap - nf) has room
+    // for the m that move (source and destination ranges are disjoint).
+    const int gap = cap - nn - nf;
+    const int m = nf < gap ? nf : gap;
+    for (int q = l; q < m; q += 32) nbr[ell(i, nn + q, cap)] = nbr[ell(i, cap - 1 - q, cap)];
     const int n = nn + nf;  // == min(count, cap)
     if (l == 0) {
       cnt[i] = n;
