@@ -45,7 +45,7 @@ using namespace sched;
 constexpr int kMorseLanes = 8;
 constexpr int kMorseSplitMax = 131072;
 // The fused rebuild's list search takes 32 lanes per atom up to this size.
-constexpr int kNlistSplitMax = 65536;
+constexpr int kNlistSplitMax = 131072;
 
 // Operation codes of a step sequence (also the graph-cache key alphabet).
 enum Op : char {
