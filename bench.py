@@ -12,7 +12,10 @@ One JSON line on stdout: the mixed-precision throughput (value), the FP64
 path's throughput on the same workload, the force error of the mixed path
 against the FP64 CPU oracle, the NVE energy drift, a roofline object for the
 dominant kernel (the EAM force pass) and a CPU baseline (the oracle's FP64
-driver loop on one core, bounded sample).
+driver loop on one core, bounded sample). Under "extras": the 32,000-atom
+Morse NVE workload, the reference's own tiny / small / medium systems beside
+its published timesteps per second, 64 batched alloy replicas, and force
+matching with the adjoint gradient.
 """
 from __future__ import annotations
 

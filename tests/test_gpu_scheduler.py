@@ -295,16 +295,21 @@ def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged():
     assert np.abs(d).max() < 1e-9
 
 
+@pytest.mark.parametrize("pot", ["johnson", "morse13"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_fused_rebuild_equals_separate_kernels(prec):
+def test_fused_rebuild_equals_separate_kernels(prec, pot):
     # the one-launch rebuild and the nine-kernel rebuild build the same sort
-    # and the same rows: trajectories are bitwise identical
-    s = fcc(6)
-    say("fused rebuild", prec)
+    # and the same rows: trajectories are bitwise identical. Systems this
+    # small take the 32-lane list search in the fused rebuild and one lane
+    # per atom in the separate kernels. morse13: a 13 A cutoff, ~290 atoms
+    # per cell (several 32-candidate rounds per cell) and ~1000-slot rows.
+    s = fcc(6) if pot == "johnson" else fcc(13)
+    params = potential("johnson") if pot == "johnson" else MORSE_CU[:3] + (13.0,)
+    say("fused rebuild", prec, pot)
     out = []
     for split in (False, True):
         cfg = FastPipelineConfig(precision=prec, rebuild_every=5, split_rebuild=split)
-        sched = FastPipelineScheduler.for_system(s, potential("johnson"), cfg)
+        sched = FastPipelineScheduler.for_system(s, params, cfg)
         sched.run_until(30)
         pos, vel, f, _, _ = sched.download()
         out.append((pos, vel, f, sched.stats()))
