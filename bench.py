@@ -330,6 +330,8 @@ def main() -> int:
                     help="force-matching Adam iterations (0: skip the force-matching leg)")
     ap.add_argument("--fm-only", action="store_true",
                     help="run only the force-matching leg and print its JSON (child process)")
+    ap.add_argument("--ref-only", action="store_true",
+                    help="run only the reference-size legs and print their JSON")
     ap.add_argument("--profile", action="store_true",
                     help="short mixed run only (for rocprofv3)")
     args = ap.parse_args()
@@ -347,6 +349,11 @@ def main() -> int:
         # binds to the same one in this process
         import torch  # noqa: F401
         print(json.dumps(bench_force_matching(args.fm_iters)), flush=True)
+        return 0
+
+    if args.ref_only:
+        capi.call("tdmd_set_device", 0)
+        print(json.dumps(bench_reference_sizes(args.ref_steps)), flush=True)
         return 0
 
     capi.call("tdmd_set_device", 0)
