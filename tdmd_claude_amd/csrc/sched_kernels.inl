@@ -1362,8 +1362,9 @@ constexpr int kRebuildBlock = 1024;  // workgroup of the fused rebuild
 template <class R, class Mir>
 __global__ void __launch_bounds__(kRebuildBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double near_sq,
-                int32_t* nbr, int32_t* cnt, GridBarrier* bar, int64_t* rebuilds) {
-  if (!a.flags[0]) return;  // uniform: nothing writes the flag before the last phase
+                int32_t* nbr, int32_t* cnt, GridBarrier* bar, int64_t* rebuilds, int forced) {
+  // uniform: nothing writes the flag before the last phase
+  if (!forced && !a.flags[0]) return;
   const int tid = blockIdx.x * kRebuildBlock + threadIdx.x;
   const int nthr = gridDim.x * kRebuildBlock;
   const int N = a.natoms;

@@ -483,12 +483,14 @@ struct Engine final : EngineBase {
   }
 
   // Conditional rebuild: one fused launch, or the separate kernels.
-  int rebuild(hipStream_t s) {
+  // forced: rebuild whatever the flag says (the fused kernel takes it as an
+  // argument; the separate kernels need the flag raised first)
+  int rebuild(hipStream_t s, bool forced) {
     if (S->nb_rebuild > 0) {
       k_rebuild_fused<R, Mir><<<S->nb_rebuild, kRebuildBlock, 0, s>>>(
           S->a, S->g, S->t, S->r_list * S->r_list, S->cutoff * S->cutoff,
           const_cast<int32_t*>(S->a.nbr), const_cast<int32_t*>(S->a.cnt), S->d_bar,
-          S->d_rebuilds);
+          S->d_rebuilds, forced ? 1 : 0);
       ++S->launches;  // the fused kernel also builds the halos
       TDMD_LAUNCH_CHECK();
       return TDMD_OK;
@@ -580,12 +582,14 @@ struct Engine final : EngineBase {
         ++S->launches;
         break;
       case OP_REB_FORCED: {
-        k_set_flag<<<1, 64, 0, s>>>(S->a.flags);
-        ++S->launches;
-        return rebuild(s);
+        if (S->nb_rebuild == 0) {
+          k_set_flag<<<1, 64, 0, s>>>(S->a.flags);
+          ++S->launches;
+        }
+        return rebuild(s, true);
       }
       case OP_REB_COND:
-        return rebuild(s);
+        return rebuild(s, false);
       case OP_DENS:
         density<false>(s);
         break;
