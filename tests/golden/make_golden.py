@@ -15,10 +15,20 @@ produced by tdmd_claude_amd.inputs, and stores inputs and outputs as .npz:
                      100 steps, thermo every 25
   ref_lattice.npz    step-0 PE/forces of tests/golden/cu_fcc_256.data for
                      Morse and Mishin EAM
+  ref_alloy2.npz     two-element EAM: a 4^3-cell FCC Ni/Cr crystal
+                     (a = 3.57 A) with random species (seed 11), U(-0.02,0.02)
+                     jitter (seed 12), 600 K (seed 13); the Ni/Cr tables of
+                     tdmd_claude_amd.inputs.johnson_alloy_setfl(nr=nrho=2000)
+                     written as setfl; 100 steps, thermo every 25
+  ref_alloy3.npz     three-element EAM: a 5^3-cell FCC Ni/Co/Cr crystal with
+                     random species (seed 21), jitter (seed 22), 900 K
+                     (seed 23); the 5000-point Ni/Co/Cr tables of
+                     johnson_alloy_setfl(); 100 steps, thermo every 25
 
 Each record: pos0, vel0, types, masses, lo, hi, f0 (step-0 forces), thermo
 rows (step, PE, KE, TE, T), pos_final, f_final, rebuilds.
-The reference binary is never shipped or run outside this container.
+Only this script writes these files; the tests read the committed .npz.
+Usage: python tests/golden/make_golden.py [--only alloy]
 """
 from __future__ import annotations
 
@@ -44,6 +54,34 @@ def c1_input() -> I.System:
     I.jitter_uniform(s, 0.02, 4242)
     I.init_velocities(s, 300.0, 42)
     return s
+
+
+ALLOY2 = ("Ni", "Cr")
+ALLOY3 = ("Ni", "Co", "Cr")
+
+
+def alloy_input(cells: int, elements, seeds) -> I.System:
+    """Random-species FCC alloy at a = 3.57 A: species, jitter and velocity
+    seeds in that order."""
+    s = I.fcc_lattice(cells, cells, cells, 3.57)
+    rng = np.random.Generator(np.random.PCG64(seeds[0]))
+    s.types = rng.integers(1, len(elements) + 1, size=s.natoms).astype(np.int32)
+    s.masses = np.array([1.0] + [I.MASSES[e] for e in elements])
+    I.jitter_uniform(s, 0.02, seeds[1])
+    I.init_velocities(s, seeds[3], seeds[2])
+    return s
+
+
+def alloy_tables(name: str):
+    if name == "alloy2":
+        return I.johnson_alloy_setfl(elements=ALLOY2, nr=2000, nrho=2000)
+    return I.johnson_alloy_setfl(elements=ALLOY3)
+
+
+def alloy_system(name: str) -> I.System:
+    if name == "alloy2":
+        return alloy_input(4, ALLOY2, (11, 12, 13, 600.0))
+    return alloy_input(5, ALLOY3, (21, 22, 23, 900.0))
 
 
 def run_ref(data: pathlib.Path, nsteps: int, thermo: int, pot_args: list[str], tmp: pathlib.Path):
@@ -80,11 +118,27 @@ def record(sys_: I.System, res: dict) -> dict:
                 lo=sys_.lo, hi=sys_.hi, **res)
 
 
+def write_alloys(tmp: pathlib.Path) -> None:
+    for name in ("alloy2", "alloy3"):
+        s = alloy_system(name)
+        data = tmp / f"{name}.data"
+        I.write_lammps_data(data, s)
+        setfl = tmp / f"{name}.eam.alloy"
+        I.write_setfl(setfl, alloy_tables(name))
+        r = run_ref(data, 100, 25, ["--eam", str(setfl)], tmp)
+        np.savez_compressed(HERE / f"ref_{name}.npz", **record(s, r))
+
+
 def main() -> None:
     if not REF.exists():
         raise SystemExit(f"{REF} missing: run `make -C oracle ref` first")
+    only = sys.argv[2] if len(sys.argv) > 2 and sys.argv[1] == "--only" else None
     with tempfile.TemporaryDirectory() as td:
         tmp = pathlib.Path(td)
+        write_alloys(tmp)
+        if only == "alloy":
+            print("golden vectors written to", HERE)
+            return
         s = c1_input()
         data = tmp / "c1.data"
         I.write_lammps_data(data, s)

@@ -388,3 +388,35 @@ def test_halo_staging_is_bitwise_neutral(r_c):
         assert info["staged"] < 7
     for k in range(3):
         assert np.array_equal(out[0][k], out[1][k])
+
+
+@pytest.mark.parametrize("name", ["alloy2", "alloy3"])
+def test_alloy_matches_reference_driver_golden(name):
+    # two- and three-element EAM on random-species crystals against the
+    # reference CPU driver's own outputs (tests/golden/ref_alloy*.npz): the
+    # FP64 scheduler's 100-step trajectory, and the mixed step-0 forces
+    from golden.make_golden import alloy_tables
+
+    g = load_golden(f"ref_{name}")
+    s = system_from_golden(g)
+    tables = alloy_tables(name)
+    nsteps = int(g["thermo"][-1, 0])
+    say("alloy vs reference golden", name, s.natoms)
+    sched = FastPipelineScheduler.for_system(
+        s, tables, FastPipelineConfig(precision="fp64", rebuild_policy="displacement"))
+    sched.run_until(nsteps)
+    pos, vel, f, _, _ = sched.download()
+    sched.close()
+    L = s.hi - s.lo
+    dpos = pos - g["pos_final"]
+    dpos -= np.round(dpos / L) * L
+    say("  fp64 max dpos", np.abs(dpos).max(), "force err", force_rel_err(f, g["f_final"]))
+    assert np.abs(dpos).max() < 1e-10
+    assert force_rel_err(f, g["f_final"]) < 1e-9
+    sched = FastPipelineScheduler.for_system(s, tables, FastPipelineConfig(precision="mixed"))
+    sched.run_until(0)
+    f0 = sched.download()[2]
+    sched.close()
+    err = force_rel_err(f0, g["f0"])
+    say("  mixed step-0 force err", err)
+    assert err < 1e-5

@@ -47,8 +47,14 @@ def test_lattice_forces_vs_lammps_dump(which):
 
 
 @pytest.mark.parametrize("name,pot", [("ref_c1_morse", "morse"), ("ref_c1_mishin", "mishin"),
-                                      ("ref_c1_johnson", "johnson")])
+                                      ("ref_c1_johnson", "johnson"), ("ref_alloy2", "alloy2"),
+                                      ("ref_alloy3", "alloy3")])
 def test_trajectory_bitexact_vs_reference_driver(name, pot):
+    # alloy2 / alloy3: two- and three-element setfl tables on random-species
+    # crystals, so the type -> element mapping (rho of the neighbour's
+    # element, F of the atom's own) and the i <= j pair-block order
+    # (eam_alloy.cpp:167-191) are pinned against the reference's reader and
+    # force loop
     g = load_golden(name)
     s = system_from_golden(g)
     th = g["thermo"]
@@ -58,6 +64,10 @@ def test_trajectory_bitexact_vs_reference_driver(name, pot):
         kw = dict(morse=MORSE_CU)
     elif pot == "mishin":
         kw = dict(eam=O.Eam(path=GOLDEN / "Cu_mishin1.eam.alloy"))
+    elif pot.startswith("alloy"):
+        from golden.make_golden import alloy_tables
+
+        kw = dict(eam=O.Eam(tables=alloy_tables(pot)))
     else:
         kw = dict(eam=O.Eam(tables=I.johnson_setfl()))
     pos, vel, f, thermo, rebuilds = O.run(s, nsteps=nsteps, thermo_every=every, **kw)
@@ -68,6 +78,19 @@ def test_trajectory_bitexact_vs_reference_driver(name, pot):
     assert np.array_equal(f, g["f_final"])
     assert rebuilds == int(g["rebuilds"])
     assert np.array_equal(O.run(s, nsteps=0, **kw)[2], g["f0"])
+
+
+@pytest.mark.parametrize("name", ["ref_alloy2", "ref_alloy3"])
+def test_alloy_golden_inputs_are_regenerated(name):
+    # the fixtures' inputs are what the generator makes (so the GPU tests can
+    # rebuild them), and every element is present
+    from golden.make_golden import alloy_system
+
+    g = load_golden(name)
+    s = alloy_system(name.split("_")[1])
+    assert np.array_equal(s.pos, g["pos0"]) and np.array_equal(s.vel, g["vel0"])
+    assert np.array_equal(s.types, g["types"]) and np.array_equal(s.masses, g["masses"])
+    assert len(np.unique(g["types"])) == len(g["masses"]) - 1
 
 
 def test_two_atom_morse_analytic():
@@ -222,6 +245,45 @@ def test_nhc_single_thermostat_closed_form():
     Q = 30 * kB * 500.0 * 0.05 ** 2
     v0 = (2 * ke - 30 * kB * 500.0) / Q * 0.001
     assert c.half_step(ke) == pytest.approx(np.exp(-v0 * 0.001), rel=1e-14)
+
+
+def _chain_run(t_target, t_period, dt, kes, M=3, ndof=765):
+    c = O.Nhc(t_target, t_period, M, dt, ndof)
+    scales = [c.half_step(ke) for ke in kes]
+    eta, eta_v, eta_m = c.state()
+    return np.array(scales), eta, eta_v, eta_m
+
+
+_KES = [12.3, 9.1, 15.7, 11.0, 8.4, 13.9]  # uneven, so every chain velocity moves
+
+
+def test_nhc_invariant_under_energy_and_temperature_scaling():
+    # (KE, T) -> (c KE, c T): every G_k is a difference of energies over a
+    # mass that scales with kT, so the chain velocities, positions and the
+    # velocity scale do not change. c = 4 is exact in binary floating point,
+    # so the half-steps must agree bit for bit; a mass or force term missing
+    # its kT (Q_1 = n_dof tau^2, G_k = (Q v^2 - 1)/Q, ...) breaks it.
+    a = _chain_run(300.0, 0.1, 0.001, _KES)
+    b = _chain_run(1200.0, 0.1, 0.001, [4.0 * ke for ke in _KES])
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert np.array_equal(4.0 * a[3], b[3])
+    assert np.all(a[2] != 0.0)
+
+
+def test_nhc_invariant_under_time_scaling():
+    # (tau, dt, v_k) -> (c tau, c dt, v_k / c): Q ~ tau^2, each kick is G dt,
+    # each friction factor exp(-v dt), so with chains started at rest the
+    # velocities come out divided by c and the positions and the velocity
+    # scale are unchanged, bit for bit for c = 2. A mass linear in tau, or
+    # a dt factor that does not pair with a 1/tau^2 or a v (dt^2, a missing
+    # half), breaks it.
+    a = _chain_run(300.0, 0.1, 0.001, _KES)
+    b = _chain_run(300.0, 0.2, 0.002, _KES)
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], 2.0 * b[2])
+    assert np.array_equal(4.0 * a[3], b[3])
 
 
 def test_nvt_holds_temperature():
