@@ -376,10 +376,12 @@ typedef struct tdmd_observables {
 } tdmd_observables;
 int tdmd_sched_observables(tdmd_sched* s, tdmd_observables* out);
 /* Timing window (no reference counterpart; measurement aid): runs `steps`
- * NVE steps without rebuild tests, with HIP events around each pass of the
+ * steps without rebuild tests, with HIP events around each pass of the
  * steady-state step, and returns the mean durations: ms_out[0] = EAM
- * density pass (Morse: kick-drift), ms_out[1] = force pass (EAM: with the
- * fused kick-drift epilogue). Advances the state like run_until. */
+ * density pass (Morse: kick-drift), ms_out[1] = force pass (EAM NVE: with
+ * the fused kick-drift epilogue; EAM NVT: with the closing half kick and the
+ * KE partials, the pass of the thermostatted step). Advances the state like
+ * run_until. NVT is accepted for EAM schedulers only. */
 int tdmd_sched_time_passes(tdmd_sched* s, int32_t steps, float* ms_out);
 /* Autotuner (no reference counterpart): times `iters` density + force
  * passes (force-only epilogue, no change to the trajectory) for each

@@ -926,9 +926,18 @@ int check_flags(tdmd_sched* S) {
     return fail(TDMD_E_NONFINITE,
                 "scheduler: non-finite position, or an atom outside the representable range "
                 "of an open axis (lo - 2W .. lo + 2W, W = max(box length, r_list))");
-  if (f[3])
+  if (f[3]) {
+    // The rebuild stopped between phases: the atoms may be half permuted and
+    // the barrier's arrival counters hold the arrivals of workgroups that
+    // returned. The kernel has ended (the stream was synchronised), so the
+    // counters are cleared here; the state is unusable until the next upload.
+    TDMD_HIP_TRY(hipMemset(S->d_bar, 0, sizeof(GridBarrier)));
+    TDMD_HIP_TRY(hipMemset(S->a.flags, 0, 32));
+    S->uploaded = false;
+    S->initial_forces = false;
     return fail(TDMD_E_HIP, "scheduler: fused rebuild grid barrier timed out (workgroups not "
-                            "co-resident); use split_rebuild");
+                            "co-resident); upload again, or use split_rebuild");
+  }
   if (f[1] > S->a.cap)
     return fail(TDMD_E_OVERFLOW, "scheduler: neighbour capacity " + std::to_string(S->a.cap) +
                                      " exceeded (max count " + std::to_string(f[1]) + ")");
@@ -1315,6 +1324,7 @@ int tdmd_sched_upload(tdmd_sched* S, const double* pos, const double* vel, const
   std::iota(iota.begin(), iota.end(), 0);
   TDMD_HIP_TRY(hipMemcpy(S->g.orig, iota.data(), N * 4, hipMemcpyHostToDevice));
   TDMD_HIP_TRY(hipMemset(S->a.flags, 0, 32));
+  if (S->d_bar) TDMD_HIP_TRY(hipMemset(S->d_bar, 0, sizeof(GridBarrier)));
   std::vector<double> nhc0(64, 0.0);  // fresh chain, no pending scale
   nhc0[24] = 1.0;
   TDMD_HIP_TRY(hipMemcpy(S->a.nhc, nhc0.data(), 64 * 8, hipMemcpyHostToDevice));
@@ -1407,34 +1417,48 @@ int tdmd_sched_interaction_cutoff(const tdmd_sched* S, double* rc) {
 }
 
 int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
-  // A window of `steps` NVE steps with HIP events around each pass of the
-  // steady-state step: EAM -> [density, force + kick-drift], Morse ->
-  // [kick-drift, force]. No rebuild test inside the window.
+  // A window of `steps` steps with HIP events around each pass of the
+  // steady-state step: EAM NVE -> [density, force + kick-drift], EAM NVT ->
+  // [density, force + half kick + KE partials] (the chain half-steps and the
+  // scaled kick-drift run between them, untimed), Morse -> [kick-drift,
+  // force]. No rebuild test inside the window.
   if (!S || !ms_out || steps < 2) return fail(TDMD_E_INVALID, "sched_time_passes: bad args");
   if (!S->uploaded) return fail(TDMD_E_STATE, "sched_time_passes: upload() first");
-  if (S->nhcp.M > 0) return fail(TDMD_E_STATE, "sched_time_passes: NVE schedulers only");
+  const bool nvt = S->nhcp.M > 0;
+  const bool eam = S->kind == 1;
+  if (nvt && !eam) return fail(TDMD_E_STATE, "sched_time_passes: NVT timing is for EAM");
   int rc;
   if (!S->initial_forces && (rc = compute_initial_forces(S))) return rc;
   hipStream_t s = S->stream;
-  const bool eam = S->kind == 1;
   const char a_op = eam ? OP_DENS : OP_KD_FULL;
-  const char b_op = eam ? OP_F_FULL : OP_F_ONLY;
+  const char b_op = nvt ? OP_F_KICK2 : eam ? OP_F_FULL : OP_F_ONLY;
   // Events are stamped by the kernel dispatches themselves
   // (hipExtLaunchKernel), so each interval is one kernel's execution.
   std::vector<hipEvent_t> ev(static_cast<size_t>(4 * steps));
   for (auto& e : ev) TDMD_HIP_TRY(hipEventCreate(&e));
-  if ((rc = S->eng->launch(OP_BOOT, s))) return rc;
-  if (!eam && (rc = S->eng->launch(OP_F_ONLY, s))) return rc;
+  if (nvt) {
+    if ((rc = S->eng->launch(OP_NVT_PRE, s))) return rc;
+  } else {
+    if ((rc = S->eng->launch(OP_BOOT, s))) return rc;
+    if (!eam && (rc = S->eng->launch(OP_F_ONLY, s))) return rc;
+  }
   for (int k = 0; k < steps - 1; ++k) {
+    if (nvt && (rc = S->eng->launch(OP_KD_NVT, s))) return rc;
     S->eng->time_next(ev[4 * k], ev[4 * k + 1]);
     if ((rc = S->eng->launch(a_op, s))) return rc;
     S->eng->time_next(ev[4 * k + 2], ev[4 * k + 3]);
     if ((rc = S->eng->launch(b_op, s))) return rc;
+    S->eng->time_next(nullptr, nullptr);
+    if (nvt && (rc = S->eng->launch(OP_NVT_POST_PRE, s))) return rc;
   }
-  S->eng->time_next(nullptr, nullptr);
-  if (eam && (rc = S->eng->launch(OP_DENS, s))) return rc;
-  if (!eam && (rc = S->eng->launch(OP_KD_FULL, s))) return rc;
-  if ((rc = S->eng->launch(OP_F_HALF, s))) return rc;
+  if (nvt) {
+    for (char op : {OP_KD_NVT, OP_DENS, OP_F_KICK2, OP_NVT_POST, OP_NVT_FINAL})
+      if ((rc = S->eng->launch(op, s))) return rc;
+  } else {
+    if (eam && (rc = S->eng->launch(OP_DENS, s))) return rc;
+    if (!eam && (rc = S->eng->launch(OP_KD_FULL, s))) return rc;
+    if ((rc = S->eng->launch(OP_F_HALF, s))) return rc;
+  }
   TDMD_HIP_TRY(hipStreamSynchronize(s));
   double ta = 0, tb = 0;
   for (int k = 0; k < steps - 1; ++k) {

@@ -176,7 +176,8 @@ class FastPipelineScheduler:
         return {name: getattr(o, name) for name, _ in capi.Observables._fields_ if name != "pad"}
 
     def time_passes(self, steps: int = 20) -> tuple[float, float]:
-        """Mean ms of the two passes of a steady-state NVE step (advances state)."""
+        """Mean ms of the two passes of a steady-state step (NVE, or NVT for EAM;
+        advances the state)."""
         out = (C.c_float * 2)()
         capi.call("tdmd_sched_time_passes", self._h, int(steps), out)
         return float(out[0]), float(out[1])
