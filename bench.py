@@ -6,16 +6,19 @@ Workload (the 1M-atom EAM configuration): FCC Cu 64x64x64 cells
 on 5000-point FD-Hermite splines (cutoff 5.5 A), skin 1.0 A with the
 skin/2 displacement rebuild, 300 K Maxwell-Boltzmann velocities (seed 7),
 NVT Nose-Hoover chain of 3 (tau 0.1 ps, 300 K), dt 1 fs. Synthetic inputs,
-state resident in HBM before the timed region.
+state resident in HBM before the timed region. Each timed window is
+preceded by the warm-up and one untimed window of the same shape, so the
+timed window only replays captured hipGraphs.
 
 One JSON line on stdout: the mixed-precision throughput (value), the FP64
-path's throughput on the same workload, the force error of the mixed path
-against the FP64 CPU oracle, the NVE energy drift, a roofline object for the
-dominant kernel (the EAM force pass) and a CPU baseline (the oracle's FP64
-driver loop on one core, bounded sample). Under "extras": the 32,000-atom
-Morse NVE workload, the reference's own tiny / small / medium systems beside
-its published timesteps per second, 64 batched alloy replicas, and force
-matching with the adjoint gradient.
+path's throughput over the same windows, the force error of the mixed path
+against the FP64 CPU oracle, the NVE energy drift (mixed and FP64, from a
+fixed thermalised state), a roofline object for the dominant kernel (the
+force pass of the timed NVT step) and a CPU baseline (the reference's own
+FP64 CPU driver, compiled from its sources, on one core). Under "extras":
+the 32,000-atom Morse NVE workload, the reference's own tiny / small /
+medium systems beside its published timesteps per second, 64 batched alloy
+replicas, and force matching with the adjoint gradient.
 """
 from __future__ import annotations
 
@@ -42,7 +45,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-THERMAL_STEPS = 1100  # NVT steps before the pass timing: the default warmup + timed steps
+THERMAL_STEPS = 1100  # NVT steps from the lattice to the state of the pass timing and drift
 
 
 def make_system(cells: int) -> I.System:
@@ -58,9 +61,13 @@ def nvt_cfg(precision: str) -> FastPipelineConfig:
 
 
 def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[float, float]:
-    """Warm up, then time exactly `steps` steps: host clock and HIP events on
-    the scheduler's stream, device synchronised on both sides."""
+    """Warm up, run one untimed window of the timed window's shape (so every
+    hipGraph the timed window replays is captured and instantiated before
+    it), then time exactly `steps` steps: host clock and HIP events on the
+    scheduler's stream, device synchronised on both sides."""
     sched.run_until(warmup)
+    sched.run_until(warmup + steps)  # graph pre-capture, untimed
+    start = warmup + steps
     stream = sched.stream()
     e0, e1 = capi.C.c_void_p(), capi.C.c_void_p()
     capi.call("tdmd_event_create", capi.C.byref(e0))
@@ -68,7 +75,7 @@ def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[fl
     capi.call("tdmd_device_synchronize")
     t0 = time.perf_counter()
     capi.call("tdmd_event_record", e0, stream)
-    sched.run_until(warmup + steps)  # ends with a stream synchronisation
+    sched.run_until(start + steps)  # ends with a stream synchronisation
     capi.call("tdmd_event_record", e1, stream)
     capi.call("tdmd_device_synchronize")
     wall = time.perf_counter() - t0
@@ -79,32 +86,99 @@ def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[fl
     return wall, ms.value / 1000.0
 
 
+def thermal_state(system: I.System, tables) -> I.System:
+    """The workload's state after THERMAL_STEPS NVT steps (mixed) from the
+    lattice: the fixed starting point of the pass timing, the drift legs and
+    the profile runs, independent of --steps and --warmup."""
+    sched = FastPipelineScheduler.for_system(system, tables, nvt_cfg("mixed"))
+    sched.run_until(THERMAL_STEPS)
+    pos, vel, _, _, _ = sched.download()
+    sched.close()
+    state = system.copy()
+    state.pos, state.vel = pos, vel
+    return state
+
+
 def bytes_per_atom(precision: str, mirror_fp64: bool, nbar: float, staged: float = 0.0,
                    halo: float = 0.0, nnear: float | None = None) -> dict:
-    """Algorithmic HBM bytes per atom of the two steady-state passes: each
-    per-atom array touched once per pass, the neighbour rows read once per
-    pass, neighbour records served from cache (L2, or LDS for staged
-    blocks). Rows are 4 bytes a slot; for the fraction `staged` of blocks
-    whose halo sits in LDS they are 2 bytes a slot plus the 4-byte halo list
-    (`halo` atoms per atom), and the force pass reads only the `nnear` slots
-    inside the cutoff at the build (its list of skin-shell slots inside the
-    cutoff, 0-3 entries, is not counted). Build positions for the skin test
-    are 32-bit coordinates with the fixed-point mirror, FP64 otherwise."""
+    """Bytes per atom of the two steady-state passes in this build's own
+    layout (the compressed model, beside the reference-layout figure of
+    alg_bytes_force): each per-atom array touched once per pass, the
+    neighbour rows read once per pass, neighbour records served from cache
+    (L2, or LDS for staged blocks). Rows are 4 bytes a slot; for the
+    fraction `staged` of blocks whose halo sits in LDS they are 2 bytes a
+    slot plus the 4-byte halo list (`halo` atoms per atom), and the force
+    pass reads only the `nnear` slots inside the cutoff at the build (its
+    list of skin-shell slots inside the cutoff, 0-3 entries, is not
+    counted). Build positions for the skin test are 32-bit coordinates with
+    the fixed-point mirror, FP64 otherwise."""
     fixed = precision == "mixed" and not mirror_fp64
     rec = 16 if fixed else 32  # M / M2 record
     xb = 12 if fixed else 24  # positions at the last build
+    fb = 12 if precision == "mixed" else 24  # forces
     nn = nbar if nnear is None else nnear
     rows_d = staged * (2 * nbar + 4 * halo) + (1 - staged) * 4 * nbar
     rows_f = staged * (2 * nn + 4 * halo) + (1 - staged) * 4 * nbar
     density = rec + 4 + rows_d + rec  # read M_i, count, rows; write M2_i
-    # force pass: read M2_i, count, type, x, v, x_build, rows; write x, v, M_i
-    force = rec + 4 + 4 + 24 + 24 + xb + rows_f + 24 + 24 + rec
-    return dict(density=density, force=force, step=density + force)
+    # NVE force pass: read M2_i, count, type, x, v, x_build, rows; write x, v, M_i
+    force_nve = rec + 4 + 4 + 24 + 24 + xb + rows_f + 24 + 24 + rec
+    # NVT force pass: read M2_i, count, type, v, rows; write F, v
+    force_nvt = rec + 4 + 4 + 24 + rows_f + fb + 24
+    return dict(density=density, force_nve=force_nve, force_nvt=force_nvt)
+
+
+def alg_bytes_force(precision: str, nbar: float) -> float:
+    """Algorithmic bytes per atom of the EAM force pass in the reference's
+    data layout: read x (24), type (4), F'(rho) (4 / 8), the row offset and
+    count (8) and the int32 full-list row (4 nbar); write F (12 / 24)."""
+    return (52.0 if precision == "mixed" else 68.0) + 4.0 * nbar
+
+
+def alg_bytes_step(precision: str, nbar: float, n_rb: float) -> float:
+    """Algorithmic bytes per atom-step of the whole EAM step in the
+    reference's layout: kick-drift, density, embedding and force passes, and
+    the list rebuild spread over the n_rb steps between rebuilds
+    (220 + 8 nbar + (104 + 4 nbar) / n_rb mixed, 252 + ... FP64)."""
+    base = 220.0 if precision == "mixed" else 252.0
+    return base + 8.0 * nbar + (104.0 + 4.0 * nbar) / max(n_rb, 1.0)
+
+
+def energy_slope(t_ps, e, discard: float = 0.25) -> float:
+    """Least-squares slope of total energy per atom against time after the
+    first `discard` of the run (the transient), in eV/atom/ns: the method of
+    the reference's NVE drift check (verifylab/cases/nve-drift/check.py)."""
+    t = np.asarray(t_ps)
+    e = np.asarray(e)
+    keep = t >= t[0] + discard * (t[-1] - t[0])
+    return float(np.polyfit(t[keep], e[keep], 1)[0] * 1000.0)
+
+
+def nve_drift(system: I.System, potential, precision: str, steps: int, every: int) -> dict:
+    """NVE from the given state (skin/2 displacement rebuild): total energy
+    per atom sampled every `every` steps, slope after a 25 % transient."""
+    cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=precision,
+                             rebuild_policy="displacement")
+    sched = FastPipelineScheduler.for_system(system, potential, cfg)
+    t, e = [], []
+    for k in range(0, steps + 1, every):
+        sched.run_until(k)
+        th = sched.thermo()
+        t.append(k * 0.001)
+        e.append((th["pe"] + th["ke"]) / system.natoms)
+    rebuilds = sched.stats().rebuilds
+    sched.close()
+    e = np.array(e)
+    return dict(drift_ev_per_atom_per_ns=energy_slope(t, e),
+                max_dev_ev_per_atom=float(np.abs(e - e[0]).max()), steps=steps,
+                sample_every=every, discarded="first 25 %", precision=precision,
+                rebuilds=rebuilds)
 
 
 def bench_morse_c2(steps: int) -> dict:
     """32,000-atom Morse Cu (20^3 cells, 300 K, +-0.02 A jitter), NVE, skin/2
-    displacement rebuild: mixed and FP64 throughput and the mixed energy drift."""
+    displacement rebuild: mixed and FP64 throughput, and the energy drift of
+    each over a separate run of the same length (energy every 500 steps,
+    slope after a 25 % transient)."""
     s = I.fcc_lattice(20, 20, 20, 3.615)
     I.init_velocities(s, 300.0, 42)
     I.jitter_uniform(s, 0.02, 4242)
@@ -115,18 +189,14 @@ def bench_morse_c2(steps: int) -> dict:
         cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=prec,
                                  rebuild_policy="displacement")
         sched = FastPipelineScheduler.for_system(s, morse, cfg)
-        sched.run_until(0)
-        th0 = sched.thermo()
         wall, _ = timed_run(sched, 100, steps)
-        th1 = sched.thermo()
-        e0 = (th0["pe"] + th0["ke"]) / s.natoms
-        e1 = (th1["pe"] + th1["ke"]) / s.natoms
         out[prec] = {"value": round(s.natoms * steps / wall / 1e6, 2),
                      "ms_per_step": round(1000 * wall / steps, 4),
-                     "energy_change_ev_per_atom": e1 - e0,
-                     "drift_ev_per_atom_per_ns": (e1 - e0) / ((steps + 100) * 1e-6),
                      "rebuilds": sched.stats().rebuilds}
         sched.close()
+        d = nve_drift(s, morse, prec, steps, 500)
+        out[prec]["drift_ev_per_atom_per_ns"] = d["drift_ev_per_atom_per_ns"]
+        out[prec]["drift_max_dev_ev_per_atom"] = d["max_dev_ev_per_atom"]
     out["mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
     return out
 
@@ -274,40 +344,153 @@ def bench_force_matching(iters: int) -> dict:
     }
 
 
-def measured_traffic(natoms: int, mirror_fp64: bool):
-    """HBM bytes per launch of the force pass from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (profiles/eam_passes_1M_mixed_traffic.json),
-    when they were taken on this workload; else None."""
-    path = os.path.join(ROOT, "profiles", "eam_passes_1M_mixed_traffic.json")
-    if mirror_fp64 or not os.path.exists(path):
+def profile_doc(name: str, natoms: int):
+    """A committed rocprofv3 summary under profiles/ (JSON), when it was taken
+    on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
         return None
     with open(path) as fh:
         doc = json.load(fh)
-    if doc.get("natoms") != natoms:
+    return doc if doc.get("natoms") == natoms else None
+
+
+def measured_traffic(natoms: int, mirror_fp64: bool, kernel: str):
+    """HBM bytes per launch of a pass from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (profiles/eam_passes_1M_mixed_traffic.json)."""
+    doc = None if mirror_fp64 else profile_doc("eam_passes_1M_mixed_traffic.json", natoms)
+    if doc is None or kernel not in doc["kernels"]:
         return None
-    k = doc["kernels"]["force_nve"]
+    k = doc["kernels"][kernel]
     return {"bytes": round(k["hbm_bytes"]), "bytes_per_atom": round(k["bytes_per_atom"], 1),
             "source": "profiles/eam_passes_1M_mixed_traffic.json"}
 
 
-def nve_drift(system: I.System, tables, precision: str, steps: int, every: int) -> dict:
-    """Mixed NVE run from the given state: slope of total energy per atom."""
-    cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=precision,
-                             rebuild_policy="displacement")
-    sched = FastPipelineScheduler.for_system(system, tables, cfg)
-    t, e = [], []
-    for k in range(0, steps + 1, every):
-        sched.run_until(k)
-        th = sched.thermo()
-        t.append(k * 0.001)
-        e.append((th["pe"] + th["ke"]) / system.natoms)
-    t = np.array(t)
-    e = np.array(e)
-    slope = np.polyfit(t, e, 1)[0]  # eV/atom/ps
-    sched.close()
-    return dict(drift_ev_per_atom_per_ns=float(slope * 1000.0),
-                max_dev_ev_per_atom=float(np.abs(e - e[0]).max()), steps=steps,
-                precision=precision)
+def measured_flops(natoms: int, mirror_fp64: bool, kernel: str):
+    """FP32 and FP64 operations per launch of a pass from the committed
+    rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F{32,64} runs
+    (profiles/eam_passes_1M_mixed_valu.json): 64 lanes per wave-level
+    instruction, two operations per FMA."""
+    doc = None if mirror_fp64 else profile_doc("eam_passes_1M_mixed_valu.json", natoms)
+    if doc is None or kernel not in doc["kernels"]:
+        return None
+    k = doc["kernels"][kernel]
+    return {"fp32_ops": k["fp32_ops"], "fp64_ops": k["fp64_ops"],
+            "source": "profiles/eam_passes_1M_mixed_valu.json"}
+
+
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "tdmd_standalone")
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_reference(cells: int, steps: int) -> dict:
+    """The reference's own CPU driver (src/drivers/tdmd_main.cpp, FP64, one
+    thread), compiled from the reference sources into oracle/_ref by
+    oracle/Makefile, on the workload's input: the lattice with its 300 K
+    velocities and the same EAM tables as a setfl file. Two runs, 0 and
+    `steps` steps; their difference is the cost of the steps (the input
+    parsing, the first list build and the first forces cancel)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    system = make_system(cells)
+    N = system.natoms
+    tmp = tempfile.mkdtemp(prefix="tdmd_cpu_")
+    try:
+        data = os.path.join(tmp, "c3.data")
+        setfl = os.path.join(tmp, "c3.eam.alloy")
+        I.write_lammps_data(data, system)
+        I.write_setfl(setfl, I.johnson_setfl())
+        cpus = sorted(os.sched_getaffinity(0))
+        os.sched_setaffinity(0, {cpus[-1]})  # this process and the driver: one core
+        times, rebuilds = [], []
+        for n in (0, steps):
+            cmd = [REF_DRIVER, "--data", data, "--eam", setfl, "--nsteps", str(n),
+                   "--thermo", str(max(n, 1))]
+            t0 = time.perf_counter()
+            res = subprocess.run(cmd, capture_output=True, text=True, timeout=1800)
+            times.append(time.perf_counter() - t0)
+            if res.returncode != 0:
+                raise RuntimeError(f"reference driver failed: {res.stderr[-500:]}")
+            reb = [ln for ln in res.stdout.splitlines() if ln.startswith("Neighbor list")]
+            rebuilds.append(int(reb[-1].split()[-1]) if reb else -1)
+            log(f"cpu baseline: reference driver, {n} steps, {times[-1]:.1f} s")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    per_step = (times[1] - times[0]) / steps
+    return dict(
+        value=N / per_step / 1e6, unit="Matom-steps/s", cores=1, kind="reference",
+        sample=(f"{N}-atom EAM Cu lattice with 300 K velocities (the workload's input), {steps} "
+                "NVE steps of the reference driver tdmd_standalone (FP64, built from "
+                "src/drivers/tdmd_main.cpp and its CPU sources), timed as the difference of a "
+                f"{steps}-step and a 0-step run: {per_step:.2f} s/step; its skin test rebuilt "
+                f"the list {rebuilds[1] - rebuilds[0]} times in {steps} steps (it measures "
+                "displacement without the minimum image, so an atom crossing the periodic "
+                "boundary triggers a rebuild); one core of "
+                f"{len(cpus)}, {cpu_model()}, in its own process beside the GPU legs"),
+        cpu_model=cpu_model(), host_cpus=len(cpus), s_per_step=per_step)
+
+
+class CpuBaseline:
+    """The CPU baseline leg. With the reference driver built, cpu_reference()
+    runs in a child process on one core, started before the GPU legs and
+    joined after them (its own interpreter: no lock shared with the timed
+    GPU windows). Otherwise the oracle's C restatement of the same loop
+    (kind "port") is timed in this process at the end."""
+
+    def __init__(self, system: I.System, tables, cells: int, steps: int):
+        import subprocess
+
+        self.system, self.tables, self.steps = system, tables, steps
+        self.proc = None
+        if os.access(REF_DRIVER, os.X_OK):
+            self.proc = subprocess.Popen(
+                [sys.executable, os.path.abspath(__file__), "--cpu-only", "--cells", str(cells),
+                 "--cpu-steps", str(steps)], stdout=subprocess.PIPE, text=True)
+
+    def join(self):
+        import subprocess
+
+        if self.proc is None:
+            return self._run_port()
+        while True:
+            try:
+                out, _ = self.proc.communicate(timeout=30)
+                break
+            except subprocess.TimeoutExpired:
+                log("cpu baseline: reference driver still running")
+        if self.proc.returncode != 0:
+            raise RuntimeError(f"cpu baseline failed (exit {self.proc.returncode})")
+        return json.loads(out.strip().splitlines()[-1])
+
+    def _run_port(self):
+        from oracle import oracle as O
+
+        eam = O.Eam(tables=self.tables)
+        N = self.system.natoms
+        t0 = time.perf_counter()
+        O.run(self.system, eam=eam, nsteps=0, thermo_every=1)
+        t1 = time.perf_counter()
+        O.run(self.system, eam=eam, nsteps=self.steps, thermo_every=self.steps)
+        t2 = time.perf_counter()
+        per_step = ((t2 - t1) - (t1 - t0)) / self.steps
+        return dict(value=N / per_step / 1e6, unit="Matom-steps/s", cores=1, kind="port",
+                    sample=(f"{N}-atom EAM Cu lattice with 300 K velocities, {self.steps} steps "
+                            "of the oracle's FP64 driver loop (the reference driver was not "
+                            f"built here), {per_step:.2f} s/step; one core, {cpu_model()}"),
+                    cpu_model=cpu_model(), host_cpus=len(os.sched_getaffinity(0)),
+                    s_per_step=per_step)
 
 
 def main() -> int:
@@ -316,8 +499,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cells", type=int, default=64)
-    ap.add_argument("--fp64-steps", type=int, default=500)
-    ap.add_argument("--nve-steps", type=int, default=2000)
+    ap.add_argument("--nve-steps", type=int, default=20000,
+                    help="NVE steps of each 1M-atom drift leg (mixed and FP64)")
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="steps of the CPU baseline (reference driver)")
     ap.add_argument("--mirror-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=10000,
@@ -330,6 +515,8 @@ def main() -> int:
                     help="force-matching Adam iterations (0: skip the force-matching leg)")
     ap.add_argument("--fm-only", action="store_true",
                     help="run only the force-matching leg and print its JSON (child process)")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="run only the reference CPU driver leg and print its JSON (child process)")
     ap.add_argument("--ref-only", action="store_true",
                     help="run only the reference-size legs and print their JSON")
     ap.add_argument("--profile", action="store_true",
@@ -351,18 +538,26 @@ def main() -> int:
         print(json.dumps(bench_force_matching(args.fm_iters)), flush=True)
         return 0
 
+    if args.cpu_only:
+        print(json.dumps(cpu_reference(args.cells, args.cpu_steps)), flush=True)
+        return 0
+
     if args.ref_only:
         capi.call("tdmd_set_device", 0)
         print(json.dumps(bench_reference_sizes(args.ref_steps)), flush=True)
         return 0
 
+    tables = I.johnson_setfl()
+    system = make_system(args.cells)
+    N = system.natoms
+    cpu_job = None
+    if not args.no_cpu_baseline and not args.profile:
+        cpu_job = CpuBaseline(system, tables, args.cells, args.cpu_steps)
+
     capi.call("tdmd_set_device", 0)
     name = capi.C.create_string_buffer(256)
     capi.call("tdmd_device_name", name, 256)
     log("device:", name.value.decode())
-    tables = I.johnson_setfl()
-    system = make_system(args.cells)
-    N = system.natoms
     log(f"system: {N} atoms, box {system.hi[0]:.2f} A")
 
     if args.profile:
@@ -378,17 +573,14 @@ def main() -> int:
                                                   "mirror_fp64": args.mirror_fp64}))
         wall, dev = timed_run(sched, args.warmup, args.steps)
         log(f"profile run: {args.steps} steps in {wall:.3f} s")
-        # on to the state the benchmark's pass timing sees (1100 NVT steps)
-        sched.run_until(max(THERMAL_STEPS, args.warmup + args.steps))
-        pos_t, vel_t, _, _, _ = sched.download()
         sched.close()
-        state = system.copy()
-        state.pos, state.vel = pos_t, vel_t
-        nve = FastPipelineScheduler.for_system(
-            state, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
-        ta, tb = nve.time_passes(30)
-        log(f"passes (thermalised state): density {ta:.4f} ms, force {tb:.4f} ms")
-        nve.close()
+        state = thermal_state(system, tables)
+        for label, cfg in (("NVT", nvt_cfg("mixed")), ("NVE", FastPipelineConfig())):
+            cfg.mirror_fp64 = args.mirror_fp64
+            p = FastPipelineScheduler.for_system(state, tables, cfg)
+            ta, tb = p.time_passes(30)
+            log(f"{label} passes (thermalised state): density {ta:.4f} ms, force {tb:.4f} ms")
+            p.close()
         # a list rebuild every 5 steps: the fused rebuild's own cost in a trace
         reb = FastPipelineScheduler.for_system(
             state, tables, FastPipelineConfig(precision="mixed", rebuild_policy="fixed",
@@ -401,49 +593,62 @@ def main() -> int:
     cfg = nvt_cfg("mixed")
     cfg.mirror_fp64 = args.mirror_fp64
     sched = FastPipelineScheduler.for_system(system, tables, cfg)
-    log("mixed: warmup + timed run")
+    log("mixed: warmup, graph pre-capture window, timed window")
     wall, dev = timed_run(sched, args.warmup, args.steps)
     th = sched.thermo()
     value = N * args.steps / wall / 1e6
-    log(f"mixed: {args.steps} steps, wall {wall:.4f} s (events {dev:.4f} s) -> {value:.1f} "
-        f"Matom-steps/s; T {th['temp']:.1f} K, mean neighbours {th['mean_nbr']:.2f}")
     stats = sched.stats()
+    log(f"mixed: {args.steps} steps, wall {wall:.4f} s (events {dev:.4f} s) -> {value:.1f} "
+        f"Matom-steps/s; T {th['temp']:.1f} K, mean neighbours {th['mean_nbr']:.2f}, "
+        f"rebuilds {stats.rebuilds}")
     pos_m, vel_m, f_m, _, _ = sched.download()
     sched.close()
 
-    # ---- per-pass timing of the steady-state NVE step (roofline) ----
-    state = system.copy()
-    state.pos, state.vel = pos_m, vel_m
+    # ---- FP64 path, same workload, same windows ----
+    s64 = FastPipelineScheduler.for_system(system, tables, nvt_cfg("fp64"))
+    log("fp64: warmup, graph pre-capture window, timed window")
+    w64, d64 = timed_run(s64, args.warmup, args.steps)
+    v64 = N * args.steps / w64 / 1e6
+    fp64 = dict(value=v64, steps=args.steps, ms_per_step=1000 * w64 / args.steps,
+                rebuilds=s64.stats().rebuilds)
+    s64.close()
+    log(f"fp64: {v64:.1f} Matom-steps/s")
+
+    # ---- per-pass timing of the benchmark's NVT step on the thermalised state ----
+    state = thermal_state(system, tables)
+    ncfg = nvt_cfg("mixed")
+    ncfg.mirror_fp64 = args.mirror_fp64
+    nvt = FastPipelineScheduler.for_system(state, tables, ncfg)
+    nvt.run_until(0)
+    obs = nvt.observables()
+    nbar, nnear = obs["mean_neighbors"], obs["mean_near_neighbors"]
+    t_dens, t_force = nvt.time_passes(30)
+    halo = nvt.halo_info()
+    nvt.close()
     nve = FastPipelineScheduler.for_system(
         state, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
-    nve.run_until(0)
-    obs = nve.observables()
-    nbar, nnear = obs["mean_neighbors"], obs["mean_near_neighbors"]
-    t_dens, t_force = nve.time_passes(30)
-    halo = nve.halo_info()
+    _, t_force_nve = nve.time_passes(30)
     nve.close()
     staged = halo["staged"] / halo["blocks"] if halo["blocks"] else 0.0
     halo_pa = halo["halo_atoms"] / N if halo["blocks"] else 0.0
     bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar, staged, halo_pa, nnear)
-    traffic = measured_traffic(N, args.mirror_fp64)
-    achieved = N * bpa["force"] / (t_force * 1e-3) / 1e9
-    log(f"passes: density {t_dens:.4f} ms, force {t_force:.4f} ms; force pass "
-        f"{achieved:.0f} GB/s algorithmic ({bpa['force']:.0f} B/atom)")
+    b_alg = alg_bytes_force("mixed", nbar)
+    achieved = N * b_alg / (t_force * 1e-3) / 1e9
+    traffic = measured_traffic(N, args.mirror_fp64, "force_nvt")
+    flops = measured_flops(N, args.mirror_fp64, "force_nvt")
+    log(f"NVT passes: density {t_dens:.4f} ms, force {t_force:.4f} ms (NVE force "
+        f"{t_force_nve:.4f} ms); force pass {achieved:.0f} GB/s on {b_alg:.1f} B/atom")
+    # steps between rebuilds over the whole mixed run (warmup, pre-capture and
+    # timed windows; the first build is not a rebuild)
+    run_steps = args.warmup + 2 * args.steps
+    n_rb = run_steps / (stats.rebuilds - 1) if stats.rebuilds > 1 else float(run_steps)
 
-    # ---- FP64 path, same workload ----
-    fp64 = None
-    if args.fp64_steps > 0:
-        s64 = FastPipelineScheduler.for_system(system, tables, nvt_cfg("fp64"))
-        log("fp64: warmup + timed run")
-        w64, d64 = timed_run(s64, args.warmup, args.fp64_steps)
-        v64 = N * args.fp64_steps / w64 / 1e6
-        s64.close()
-        fp64 = dict(value=v64, steps=args.fp64_steps, ms_per_step=1000 * w64 / args.fp64_steps)
-        log(f"fp64: {v64:.1f} Matom-steps/s")
-
-    # ---- NVE energy drift (mixed) ----
-    drift = nve_drift(state, tables, "mixed", args.nve_steps, max(args.nve_steps // 10, 1))
-    log(f"NVE drift: {drift['drift_ev_per_atom_per_ns']:.3e} eV/atom/ns")
+    # ---- NVE energy drift from the thermalised state, mixed and FP64 ----
+    drift = {}
+    for prec in ("mixed", "fp64"):
+        drift[prec] = nve_drift(state, tables, prec, args.nve_steps, 500)
+        log(f"NVE drift {prec}: {drift[prec]['drift_ev_per_atom_per_ns']:.3e} eV/atom/ns "
+            f"over {args.nve_steps} steps")
 
     # ---- other workloads of the path (not the metric) ----
     extras = {}
@@ -478,33 +683,20 @@ def main() -> int:
             f"{fmx['eam']['eval_ms']} ms, EAM fit {fmx['eam']['fit_s']} s "
             f"(loss {fmx['eam']['loss_start']:.4e} -> {fmx['eam']['loss_end']:.4e})")
 
-    # ---- CPU baseline leg: FP64 oracle driver (also the force checker) ----
-    cpu = None
-    force_err = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle as O
+    # ---- force check against the FP64 oracle (the checker), then the CPU baseline ----
+    from oracle import oracle as O
 
-        eam = O.Eam(tables=tables)
-        ref_state = system.copy()
-        ref_state.pos = pos_m
-        ref_state.vel = vel_m
-        t0 = time.perf_counter()
-        _, _, f_ref, _, _ = O.run(ref_state, eam=eam, nsteps=0, thermo_every=1)
-        t1 = time.perf_counter()
-        rms = float(np.sqrt(np.mean(np.sum(f_ref * f_ref, axis=1))))
-        force_err = float(np.max(np.linalg.norm(f_m - f_ref, axis=1)) / rms)
-        log(f"force error vs FP64 oracle: {force_err:.3e} (max |dF| / RMS |F|, RMS {rms:.3f})")
-        nsteps_cpu = 2
-        t2 = time.perf_counter()
-        O.run(ref_state, eam=eam, nsteps=nsteps_cpu, thermo_every=nsteps_cpu)
-        t3 = time.perf_counter()
-        per_step = ((t3 - t2) - (t1 - t0)) / nsteps_cpu
-        cpu = dict(value=N / per_step / 1e6, unit="Matom-steps/s", cores=1, kind="port",
-                   sample=(f"{N}-atom EAM Cu (the GPU run's state), {nsteps_cpu} steps of the "
-                           "FP64 driver loop (tdmd_main.cpp:237-254) after its list build; "
-                           f"{per_step:.2f} s/step on {os.cpu_count()} host CPUs, 1 used"))
-        log(f"cpu baseline: {cpu['value']:.4f} Matom-steps/s")
+    ref_state = system.copy()
+    ref_state.pos, ref_state.vel = pos_m, vel_m
+    _, _, f_ref, _, _ = O.run(ref_state, eam=O.Eam(tables=tables), nsteps=0, thermo_every=1)
+    rms = float(np.sqrt(np.mean(np.sum(f_ref * f_ref, axis=1))))
+    force_err = float(np.max(np.linalg.norm(f_m - f_ref, axis=1)) / rms)
+    log(f"force error vs FP64 oracle: {force_err:.3e} (max |dF| / RMS |F|, RMS {rms:.3f})")
+    cpu = cpu_job.join() if cpu_job is not None else None
+    if cpu is not None:
+        log(f"cpu baseline ({cpu['kind']}): {cpu['value']:.4f} Matom-steps/s")
 
+    step_bytes = alg_bytes_step("mixed", nbar, n_rb)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -527,25 +719,39 @@ def main() -> int:
             "mean_neighbours": round(nbar, 2),
             "rebuilds": stats.rebuilds,
             "kernel_launches_per_step": round(stats.kernel_launches / max(stats.ticks, 1), 2),
+            "untimed_precapture_steps": args.steps,
         },
         "events_s": round(dev, 5),
-        "fp64": None if fp64 is None else {k: round(v, 4) if isinstance(v, float) else v
-                                           for k, v in fp64.items()},
-        "mixed_over_fp64": None if fp64 is None else round(value / fp64["value"], 3),
+        "fp64": {k: round(v, 4) if isinstance(v, float) else v for k, v in fp64.items()},
+        "mixed_over_fp64": round(value / fp64["value"], 3),
         "force_rel_err_vs_fp64_oracle": force_err,
-        "nve_energy_drift": drift,
+        "nve_energy_drift": {
+            "drift_ev_per_atom_per_ns": drift["mixed"]["drift_ev_per_atom_per_ns"],
+            "start": f"the state after {THERMAL_STEPS} NVT steps from the lattice",
+            "mixed": drift["mixed"], "fp64": drift["fp64"]},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_eam_force_lds<float, MirrorFixed> (force + fused NVE kick-drift)",
+            "kernel": "k_eam_force_lds<float, MirrorFixed, 3, ...> (force + closing half kick "
+                      "+ KE partials: the force pass of the timed NVT step)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None if traffic is None else traffic["bytes"],
             "traffic_per_atom": None if traffic is None else traffic["bytes_per_atom"],
-            "bytes_per_atom": round(bpa["force"], 1),
+            "bytes_per_atom": round(b_alg, 1),
+            "bytes_per_atom_rule": "52 + 4 nbar: x, type, F'(rho), row offset and count, "
+                                   "int32 full-list row, F (the reference's layout)",
+            "bytes_per_atom_own_layout": round(bpa["force_nvt"], 1),
             "avg_ms": round(t_force, 5),
             "density_pass_ms": round(t_dens, 5),
+            "nve_force_pass_ms": round(t_force_nve, 5),
+            "fp32_tflops": None if flops is None else round(flops["fp32_ops"] / (t_force * 1e9), 2),
+            "fp64_tflops": None if flops is None else round(flops["fp64_ops"] / (t_force * 1e9), 2),
+            "fp32_peak_tflops": 157.3,
+            "fp64_peak_tflops": 78.6,
+            "step_bytes_per_atom": round(step_bytes, 1),
+            "step_achieved": round(N * step_bytes * args.steps / wall / 1e9, 1),
             "halo_staged_blocks": f"{halo['staged']}/{halo['blocks']}",
             "halo_atoms_per_atom": round(halo_pa, 3),
             "near_slots_per_atom": round(nnear, 2),
