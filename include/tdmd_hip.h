@@ -297,7 +297,11 @@ typedef struct {
                           /* 0 (default): one fused launch per step       */
   int32_t disable_halo;   /* 1: no LDS halo staging in the single-element */
                           /* EAM passes (0: staged where records are 16 B)*/
-  int32_t reserved[4];
+  int32_t pair_lanes;     /* lanes per neighbour row in the force passes:  */
+                          /* 0 = by size (8 up to 131072 atoms, else 1),  */
+                          /* 1, 4 or 8; EAM with more than one lane runs  */
+                          /* the global-table passes (no LDS tables/halo) */
+  int32_t reserved[3];
 } tdmd_sched_config;
 
 void tdmd_sched_config_default(tdmd_sched_config* cfg);

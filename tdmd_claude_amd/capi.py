@@ -85,7 +85,8 @@ class SchedConfig(C.Structure):
         ("lds_budget", C.c_int32),
         ("split_rebuild", C.c_int32),
         ("disable_halo", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("pair_lanes", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

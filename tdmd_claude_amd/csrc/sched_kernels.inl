@@ -381,99 +381,6 @@ __device__ __forceinline__ void load_meta(SplineMeta* sm, const SplineMeta* meta
   __syncthreads();
 }
 
-template <class R, class Mir, bool kEnergy>
-__global__ void __launch_bounds__(kBlock)
-k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
-              const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
-  using T = typename Mir::T;
-  __shared__ SplineMeta sm[kMaxSplines];
-  load_meta(sm, meta, 2 * a.ntypes);
-  const int blk = xcd_block(a.nblocks);
-  const int i = blk * kBlock + threadIdx.x;
-  const T* __restrict__ M = static_cast<const T*>(a.M);
-  double e = 0.0;
-  if (i < a.natoms) {
-    const T mi = M[i];
-    const MirrorParams& mpi = mp_of<Mir>(a, i);
-    const int ti = Mir::type_of(mi);
-    const int n = a.cnt[i];
-    double rho = 0.0;
-    for_neighbours<T, kPairUnroll>(a, i, n, M, [&](int, const T& mj, bool in_row) {
-      double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
-      if (in_row && r2 < rc_sq) {
-        const SplineMeta& m = sm[a.ntypes + Mir::type_of(mj)];
-        if constexpr (std::is_same_v<R, float>) {
-          float t;
-          const auto c = coeff[m.offset + locate_f32(meta_f32(m), r2, &t)];
-          rho += static_cast<double>(c.x + t * (c.y + t * (c.z + t * c.w)));
-        } else {
-          rho += spline_value<R>(m, coeff, pair_r<R>(r2).r);
-        }
-      }
-    });
-    const SplineVal<R> F = spline_both<R>(sm[ti], coeff, rho);
-    static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
-    e = static_cast<double>(F.f);
-  }
-  if (kEnergy) emit_partial(e, a.partials, blk);
-}
-
-template <class R, class Mir, int kEpi, bool kEnergy, bool kMulti>
-__global__ void __launch_bounds__(kBlock)
-k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
-            const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
-  using T = typename Mir::T;
-  __shared__ SplineMeta sm[kMaxSplines];
-  const int nt = a.ntypes;
-  load_meta(sm, meta, 2 * nt + nt * (nt + 1) / 2);
-  const int blk = xcd_block(a.nblocks);
-  const int i = blk * kBlock + threadIdx.x;
-  const T* __restrict__ M2 = static_cast<const T*>(a.M2);
-  double red = 0.0, vir = 0.0;
-  if (i < a.natoms) {
-    const T mi = M2[i];
-    const MirrorParams& mpi = mp_of<Mir>(a, i);
-    const R fpi = Mir::template w_real<R>(mi);
-    const int ti = kMulti ? a.types[i] - 1 : 0;
-    const int n = a.cnt[i];
-    double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-    for_neighbours<T, kPairUnroll>(a, i, n, M2, [&](int j, const T& mj, bool in_row) {
-      double dx, dy, dz;
-      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
-      if (in_row && r2 < rc_sq) {
-        const PairR<R> pr = pair_r<R>(r2);
-        const int tj = kMulti ? a.types[j] - 1 : 0;
-        const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
-        R dphi;
-        if (kEnergy) {
-          const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, pr.r);
-          pe += 0.5 * static_cast<double>(ph.f);
-          dphi = ph.df;
-        } else {
-          dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, pr.r);
-        }
-        const R drho_j = spline_deriv<R>(sm[nt + tj], coeff, pr.r);
-        const R drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
-        const R fpj = Mir::template w_real<R>(mj);
-        const double fpair = static_cast<double>(-(dphi + fpi * drho_j + fpj * drho_i) * pr.rinv);
-        if (kEnergy) vir += 0.5 * fpair * r2;
-        fx = fma(fpair, dx, fx);
-        fy = fma(fpair, dy, fy);
-        fz = fma(fpair, dz, fz);
-      }
-    });
-    const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
-    red = kEnergy ? pe : ke;
-  }
-  if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
-  if (kEnergy) emit_partial(vir, a.partials_w, blk);
-}
-
-// ---------------------------------------------------------------------------
-// Morse pass (morse.hpp:37-55; pair_math.hpp precision contract)
-// ---------------------------------------------------------------------------
-
 // A row split over kLanes neighbouring lanes (small systems, where one lane
 // per atom leaves most of the chip idle): lane `part` takes quads part,
 // part + kLanes, ... of the padded row; the four records of a quad are
@@ -503,6 +410,133 @@ __device__ __forceinline__ double lane_sum(double v) {
   for (int w = kLanes / 2; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
   return v;
 }
+
+// Global-table EAM passes (several elements, or systems small enough that
+// one workgroup per 1024 atoms would leave most CUs idle). kLanes > 1: the
+// row of atom i is split over kLanes neighbouring lanes (quads p, p + kLanes,
+// ... to lane p) and the lanes' partial sums are combined by a butterfly in
+// a fixed order, so the result is run-to-run identical; the grid is kLanes
+// times the one-lane grid and emits one energy / KE partial per 256 threads.
+template <class R, class Mir, bool kEnergy, int kLanes>
+__global__ void __launch_bounds__(kBlock)
+k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
+              const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
+  static_assert(kLanes >= 1 && 64 % kLanes == 0, "an atom's lanes share a wavefront");
+  using T = typename Mir::T;
+  __shared__ SplineMeta sm[kMaxSplines];
+  load_meta(sm, meta, 2 * a.ntypes);
+  const int blk = xcd_block(gridDim.x);
+  const int g = blk * kBlock + threadIdx.x;
+  const int i = g / kLanes;
+  const int part = g % kLanes;
+  const T* __restrict__ M = static_cast<const T*>(a.M);
+  double e = 0.0, rho = 0.0;
+  T mi{};
+  if (i < a.natoms) {
+    mi = M[i];
+    const MirrorParams& mpi = mp_of<Mir>(a, i);
+    const int n = a.cnt[i];
+    auto pair = [&](int, const T& mj, bool in_row) {
+      double dx, dy, dz;
+      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
+      if (in_row && r2 < rc_sq) {
+        const SplineMeta& m = sm[a.ntypes + Mir::type_of(mj)];
+        if constexpr (std::is_same_v<R, float>) {
+          float t;
+          const auto c = coeff[m.offset + locate_f32(meta_f32(m), r2, &t)];
+          rho += static_cast<double>(c.x + t * (c.y + t * (c.z + t * c.w)));
+        } else {
+          rho += spline_value<R>(m, coeff, pair_r<R>(r2).r);
+        }
+      }
+    };
+    if constexpr (kLanes == 1)
+      for_neighbours<T, kPairUnroll>(a, i, n, M, pair);
+    else
+      for_neighbours_split<T, kLanes>(a, i, n, part, M, pair);
+  }
+  if constexpr (kLanes > 1) rho = lane_sum<kLanes>(rho);  // every lane takes part
+  if (i < a.natoms && part == 0) {
+    const SplineVal<R> F = spline_both<R>(sm[Mir::type_of(mi)], coeff, rho);
+    static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
+    e = static_cast<double>(F.f);
+  }
+  if (kEnergy) emit_partial(e, a.partials, blk);
+}
+
+template <class R, class Mir, int kEpi, bool kEnergy, bool kMulti, int kLanes>
+__global__ void __launch_bounds__(kBlock)
+k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
+            const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
+  static_assert(kLanes >= 1 && 64 % kLanes == 0, "an atom's lanes share a wavefront");
+  using T = typename Mir::T;
+  __shared__ SplineMeta sm[kMaxSplines];
+  const int nt = a.ntypes;
+  load_meta(sm, meta, 2 * nt + nt * (nt + 1) / 2);
+  const int blk = xcd_block(gridDim.x);
+  const int g = blk * kBlock + threadIdx.x;
+  const int i = g / kLanes;
+  const int part = g % kLanes;
+  const T* __restrict__ M2 = static_cast<const T*>(a.M2);
+  double red = 0.0, vir = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
+  if (i < a.natoms) {
+    const T mi = M2[i];
+    const MirrorParams& mpi = mp_of<Mir>(a, i);
+    const R fpi = Mir::template w_real<R>(mi);
+    const int ti = kMulti ? a.types[i] - 1 : 0;
+    const int n = a.cnt[i];
+    auto pair = [&](int j, const T& mj, bool in_row) {
+      double dx, dy, dz;
+      const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
+      if (in_row && r2 < rc_sq) {
+        const PairR<R> pr = pair_r<R>(r2);
+        const int tj = kMulti ? a.types[j] - 1 : 0;
+        const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
+        R dphi;
+        if (kEnergy) {
+          const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, pr.r);
+          pe += 0.5 * static_cast<double>(ph.f);
+          dphi = ph.df;
+        } else {
+          dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, pr.r);
+        }
+        const R drho_j = spline_deriv<R>(sm[nt + tj], coeff, pr.r);
+        const R drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
+        const R fpj = Mir::template w_real<R>(mj);
+        const double fpair = static_cast<double>(-(dphi + fpi * drho_j + fpj * drho_i) * pr.rinv);
+        if (kEnergy) vir += 0.5 * fpair * r2;
+        fx = fma(fpair, dx, fx);
+        fy = fma(fpair, dy, fy);
+        fz = fma(fpair, dz, fz);
+      }
+    };
+    if constexpr (kLanes == 1)
+      for_neighbours<T, kPairUnroll>(a, i, n, M2, pair);
+    else
+      for_neighbours_split<T, kLanes>(a, i, n, part, M2, pair);
+  }
+  if constexpr (kLanes > 1) {  // every lane takes part in the shuffles
+    fx = lane_sum<kLanes>(fx);
+    fy = lane_sum<kLanes>(fy);
+    fz = lane_sum<kLanes>(fz);
+    if (kEnergy) {
+      pe = lane_sum<kLanes>(pe);
+      vir = lane_sum<kLanes>(vir);
+    }
+  }
+  if (i < a.natoms && part == 0) {
+    const double ke = epilogue<R, Mir, kEpi>(a, i, fx, fy, fz);
+    red = kEnergy ? pe : ke;
+  } else {
+    vir = 0.0;
+  }
+  if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
+  if (kEnergy) emit_partial(vir, a.partials_w, blk);
+}
+
+// ---------------------------------------------------------------------------
+// Morse pass (morse.hpp:37-55; pair_math.hpp precision contract)
+// ---------------------------------------------------------------------------
 
 template <class R, class Mir, int kEpi, bool kEnergy, int kLanes>
 __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p) {

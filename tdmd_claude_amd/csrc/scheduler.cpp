@@ -39,11 +39,12 @@ namespace {
 
 using namespace sched;
 
-// Morse rows of systems up to kMorseSplitMax atoms are split over
-// kMorseLanes lanes: one lane per atom would give 32,000 atoms 125
-// workgroups on 256 CUs, each lane walking ~80 slots in turn.
-constexpr int kMorseLanes = 8;
-constexpr int kMorseSplitMax = 131072;
+// Rows of systems up to kSplitMax atoms are split over kSplitLanes lanes in
+// the force passes (Morse, and EAM through the global-table passes): one
+// lane per atom would give 32,000 atoms 125 workgroups on 256 CUs, each lane
+// walking ~80 slots in turn; the LDS EAM passes would run 32 workgroups.
+constexpr int kSplitLanes = 8;
+constexpr int kSplitMax = 131072;
 // The fused rebuild's list search takes 32 lanes per atom up to this size.
 constexpr int kNlistSplitMax = 131072;
 
@@ -107,7 +108,7 @@ struct tdmd_sched {
   int natoms = 0;
   int nblocks = 0;   // 256-thread grid (multiple of 8)
   int nb_force = 0;  // grid of the EAM/Morse passes = block partials they emit
-  int morse_lanes = 1;  // lanes per atom of the Morse pass
+  int lanes = 1;  // lanes per atom of the force passes (Morse, global-table EAM)
   int nb_rebuild = 0;  // fused rebuild grid (all resident at once); 0: separate kernels
   tdmd_hip::sched::GridBarrier* d_bar = nullptr;
   tdmd_sched_config cfg{};
@@ -251,10 +252,11 @@ struct Engine final : EngineBase {
   }
 
   int init() override {
-    // Morse rows split over kMorseLanes lanes when one lane per atom would
-    // leave the chip mostly idle (partials: one per block of the larger grid)
-    S->morse_lanes = S->kind == 0 && S->natoms <= kMorseSplitMax ? kMorseLanes : 1;
-    S->nb_force = S->nblocks * S->morse_lanes;
+    // rows split over kSplitLanes lanes when one lane per atom would leave
+    // the chip mostly idle (partials: one per block of the larger grid)
+    S->lanes = S->cfg.pair_lanes > 0 ? S->cfg.pair_lanes
+                                     : (S->natoms <= kSplitMax ? kSplitLanes : 1);
+    S->nb_force = S->nblocks * S->lanes;
     S->nb_rebuild = 0;
     if (!S->cfg.split_rebuild) {
       // every workgroup of the fused rebuild must be resident at once
@@ -275,6 +277,7 @@ struct Engine final : EngineBase {
       S->g.nl_split = split && S->nb_rebuild > 0 ? 1 : 0;
     }
     if (S->kind != 1 || S->tables->ntypes != 1) return TDMD_OK;
+    if (S->lanes > 1) return TDMD_OK;  // split rows: the global-table passes
     // rho_0 and phi_00 must share the r grid (setfl: one Nr, dr).
     SplineMeta m[3];
     TDMD_HIP_TRY(hipMemcpy(m, S->tables->d_meta, sizeof(m), hipMemcpyDeviceToHost));
@@ -420,8 +423,16 @@ struct Engine final : EngineBase {
   template <bool kEnergy>
   void density(hipStream_t s) {
     if (!lds) {
-      go(k_eam_density<R, Mir, kEnergy>, S->nblocks, kBlock, 0, s, S->a, meta(), coeff(),
-         rc_sq());
+      const int nb = S->nblocks;
+      if (S->lanes == 8)
+        go(k_eam_density<R, Mir, kEnergy, 8>, nb * 8, kBlock, 0, s, S->a, meta(), coeff(),
+           rc_sq());
+      else if (S->lanes == 4)
+        go(k_eam_density<R, Mir, kEnergy, 4>, nb * 4, kBlock, 0, s, S->a, meta(), coeff(),
+           rc_sq());
+      else
+        go(k_eam_density<R, Mir, kEnergy, 1>, nb, kBlock, 0, s, S->a, meta(), coeff(),
+           rc_sq());
       return;
     }
     const bool win = w_dens[kEnergy].k_lo > 0;
@@ -453,6 +464,20 @@ struct Engine final : EngineBase {
     force_go<kEpi, kEnergy, kWin, kU, false, false>(s);
   }
 
+  template <int kEpi, bool kEnergy, bool kMulti>
+  void eam_force_global(hipStream_t s) {
+    const int nb = S->nblocks;
+    if (S->lanes == 8)
+      go(k_eam_force<R, Mir, kEpi, kEnergy, kMulti, 8>, nb * 8, kBlock, 0, s, S->a, meta(),
+         coeff(), rc_sq());
+    else if (S->lanes == 4)
+      go(k_eam_force<R, Mir, kEpi, kEnergy, kMulti, 4>, nb * 4, kBlock, 0, s, S->a, meta(),
+         coeff(), rc_sq());
+    else
+      go(k_eam_force<R, Mir, kEpi, kEnergy, kMulti, 1>, nb, kBlock, 0, s, S->a, meta(),
+         coeff(), rc_sq());
+  }
+
   template <int kEpi, bool kEnergy>
   void force(hipStream_t s) {
     const int nb = S->nblocks;
@@ -463,15 +488,14 @@ struct Engine final : EngineBase {
       else if (lds)
         win ? force_lds<kEpi, kEnergy, true, 4>(s) : force_lds<kEpi, kEnergy, false, 4>(s);
       else if (S->tables->ntypes > 1)
-        go(k_eam_force<R, Mir, kEpi, kEnergy, true>, nb, kBlock, 0, s, S->a, meta(), coeff(),
-           rc_sq());
+        eam_force_global<kEpi, kEnergy, true>(s);
       else
-        go(k_eam_force<R, Mir, kEpi, kEnergy, false>, nb, kBlock, 0, s, S->a, meta(), coeff(),
-           rc_sq());
+        eam_force_global<kEpi, kEnergy, false>(s);
     } else {
-      if (S->morse_lanes > 1)
-        go(k_morse_force<R, Mir, kEpi, kEnergy, kMorseLanes>, nb * kMorseLanes, kBlock, 0, s,
-           S->a, mp());
+      if (S->lanes == 8)
+        go(k_morse_force<R, Mir, kEpi, kEnergy, 8>, nb * 8, kBlock, 0, s, S->a, mp());
+      else if (S->lanes == 4)
+        go(k_morse_force<R, Mir, kEpi, kEnergy, 4>, nb * 4, kBlock, 0, s, S->a, mp());
       else
         go(k_morse_force<R, Mir, kEpi, kEnergy, 1>, nb, kBlock, 0, s, S->a, mp());
       if constexpr (kEpi == kFullKickDrift) {
@@ -657,9 +681,9 @@ int sched_alloc(tdmd_sched* S) {
   a.M2 = M2;
   if ((rc = dalloc(S, &a.flags, 8))) return rc;
   // room for the per-block partials of the widest pass (Morse rows split
-  // over kMorseLanes lanes)
-  if ((rc = dalloc(S, &a.partials, static_cast<size_t>(S->nblocks) * kMorseLanes + 64))) return rc;
-  if ((rc = dalloc(S, &a.partials_w, static_cast<size_t>(S->nblocks) * kMorseLanes + 64))) return rc;
+  // over kSplitLanes lanes)
+  if ((rc = dalloc(S, &a.partials, static_cast<size_t>(S->nblocks) * kSplitLanes + 64))) return rc;
+  if ((rc = dalloc(S, &a.partials_w, static_cast<size_t>(S->nblocks) * kSplitLanes + 64))) return rc;
   if ((rc = dalloc(S, &a.nhc, 64))) return rc;
   int32_t* cnt;
   if ((rc = dalloc(S, &cnt, round_up(N, 64)))) return rc;
@@ -845,6 +869,8 @@ int sched_setup(tdmd_sched* S, const std::vector<tdmd_box>& boxes, const std::ve
   if (cfg.pair_unroll != 0 && cfg.pair_unroll != 4 && cfg.pair_unroll != 8)
     return fail(TDMD_E_INVALID, "scheduler: pair_unroll must be 0 (default), 4 or 8");
   if (cfg.lds_budget < 0) return fail(TDMD_E_INVALID, "scheduler: lds_budget must be >= 0");
+  if (cfg.pair_lanes != 0 && cfg.pair_lanes != 1 && cfg.pair_lanes != 4 && cfg.pair_lanes != 8)
+    return fail(TDMD_E_INVALID, "scheduler: pair_lanes must be 0 (by size), 1, 4 or 8");
   S->box = boxes[0];
   S->natoms = natoms;
   S->cfg = cfg;

@@ -43,6 +43,7 @@ class FastPipelineConfig:
     lds_budget: int = 0  # LDS bytes per workgroup for spline windows (0 = device limit)
     split_rebuild: bool = False  # rebuild as separate kernels instead of one fused launch
     disable_halo: bool = False  # no LDS halo staging in the single-element EAM passes
+    pair_lanes: int = 0  # lanes per neighbour row: 0 = by size, 1, 4 or 8
 
     def to_c(self) -> capi.SchedConfig:
         c = capi.SchedConfig()
@@ -63,6 +64,7 @@ class FastPipelineConfig:
         c.lds_budget = self.lds_budget
         c.split_rebuild = int(self.split_rebuild)
         c.disable_halo = int(self.disable_halo)
+        c.pair_lanes = self.pair_lanes
         return c
 
 

@@ -55,11 +55,14 @@ def gpu_run(s, pot, nsteps, chunks=None, **cfg):
     return sched, pos, vel, f
 
 
+@pytest.mark.parametrize("lanes", [0, 1])
 @pytest.mark.parametrize("pot", ["morse", "johnson", "mishin"])
-def test_fp64_trajectory_matches_oracle(pot):
+def test_fp64_trajectory_matches_oracle(pot, lanes):
+    # lanes 0: rows split over 8 lanes (the default at this size; EAM then
+    # runs the global-table passes); 1: one lane per row (EAM: LDS passes)
     s = c1() if pot == "morse" else fcc(5)
-    say("sched fp64", pot, s.natoms)
-    sched, pos, vel, f = gpu_run(s, pot, 100, precision="fp64")
+    say("sched fp64", pot, s.natoms, "lanes", lanes)
+    sched, pos, vel, f = gpu_run(s, pot, 100, precision="fp64", pair_lanes=lanes)
     rpos, rvel, rf, th, reb = oracle_run(s, pot, 100)
     L = s.hi - s.lo
     dpos = pos - rpos
@@ -74,13 +77,15 @@ def test_fp64_trajectory_matches_oracle(pot):
     assert sched.potential_kind() == (PotentialKind.Morse if pot == "morse" else PotentialKind.Eam)
 
 
+@pytest.mark.parametrize("lanes", [0, 1])
 @pytest.mark.parametrize("pot", ["morse", "johnson"])
 @pytest.mark.parametrize("mirror_fp64", [False, True])
-def test_mixed_forces_and_trajectory_vs_fp64_oracle(pot, mirror_fp64):
+def test_mixed_forces_and_trajectory_vs_fp64_oracle(pot, mirror_fp64, lanes):
     s = fcc(6)
-    say("sched mixed", pot, "mirror_fp64", mirror_fp64)
+    say("sched mixed", pot, "mirror_fp64", mirror_fp64, "lanes", lanes)
     sched = FastPipelineScheduler.for_system(
-        s, potential(pot), FastPipelineConfig(precision="mixed", mirror_fp64=mirror_fp64))
+        s, potential(pot), FastPipelineConfig(precision="mixed", mirror_fp64=mirror_fp64,
+                                              pair_lanes=lanes))
     sched.run_until(0)  # initial forces only
     _, _, f0, _, _ = sched.download()
     _, _, rf0, _, _ = oracle_run(s, pot, 0)
@@ -195,13 +200,14 @@ def test_large_box_trajectory_matches_oracle(pot):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("pot,chunks", [("johnson", None), ("johnson", [7, 31, 60]),
-                                        ("morse", None), ("mishin", [25, 60])])
-def test_nvt_matches_oracle_chain(prec, pot, chunks):
+@pytest.mark.parametrize("pot,chunks,lanes", [("johnson", None, 0), ("johnson", [7, 31, 60], 0),
+                                              ("johnson", None, 1), ("morse", None, 0),
+                                              ("mishin", [25, 60], 0), ("mishin", None, 1)])
+def test_nvt_matches_oracle_chain(prec, pot, chunks, lanes):
     # windows end with the chain half-step after the step only
     s = fcc(5, T=300.0)
-    say("nvt", prec, pot, chunks)
-    cfg = dict(precision=prec, t_target=400.0, t_period=0.05, nhc_length=3)
+    say("nvt", prec, pot, chunks, "lanes", lanes)
+    cfg = dict(precision=prec, t_target=400.0, t_period=0.05, nhc_length=3, pair_lanes=lanes)
     sched, pos, vel, f = gpu_run(s, pot, 60, chunks=chunks, **cfg)
     rpos, rvel, rf, th, _ = oracle_run(s, pot, 60, nvt=True, t_target=400.0,
                                        t_period=0.05, nhc_length=3)
@@ -277,7 +283,7 @@ def test_launch_accounting():
 def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged():
     s = fcc(6)
     say("autotune")
-    cfg = FastPipelineConfig(precision="mixed")
+    cfg = FastPipelineConfig(precision="mixed", pair_lanes=1)  # the LDS passes' shapes
     ref = FastPipelineScheduler.for_system(s, potential("johnson"), cfg)
     ref.run_until(20)
     p0, v0, _, _, _ = ref.download()
@@ -373,7 +379,8 @@ def test_halo_staging_is_bitwise_neutral(r_c):
     say("halo staging", r_c)
     out = []
     for off in (False, True):
-        cfg = FastPipelineConfig(precision="mixed", rebuild_every=5, disable_halo=off)
+        cfg = FastPipelineConfig(precision="mixed", rebuild_every=5, disable_halo=off,
+                                 pair_lanes=1)
         sched = FastPipelineScheduler.for_system(s, tables, cfg)
         sched.run_until(20)
         pos, vel, f, _, _ = sched.download()
@@ -420,3 +427,35 @@ def test_alloy_matches_reference_driver_golden(name):
     err = force_rel_err(f0, g["f0"])
     say("  mixed step-0 force err", err)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("pot", ["johnson", "morse", "alloy"])
+def test_row_split_is_deterministic_and_agrees_with_one_lane(pot):
+    # 1, 4 and 8 lanes per row sum a row's pairs in different (fixed)
+    # orders: each repeats bit for bit, and the FP64 trajectories agree to
+    # rounding; the EAM one-lane run is the LDS pass, the split runs the
+    # global-table passes
+    if pot == "alloy":
+        from golden.make_golden import alloy_system, alloy_tables
+
+        s, params = alloy_system("alloy3"), alloy_tables("alloy3")
+    else:
+        s, params = fcc(6), potential(pot)
+    say("row split", pot)
+    out = {}
+    for lanes in (1, 4, 8, 8):
+        cfg = FastPipelineConfig(precision="fp64", rebuild_every=5, pair_lanes=lanes)
+        sched = FastPipelineScheduler.for_system(s, params, cfg)
+        sched.run_until(40)
+        pos, vel, f, _, _ = sched.download()
+        sched.close()
+        if lanes in out:
+            assert np.array_equal(out[lanes][0], pos) and np.array_equal(out[lanes][2], f)
+        out[lanes] = (pos, vel, f)
+    L = s.hi - s.lo
+    for lanes in (4, 8):
+        d = out[lanes][0] - out[1][0]
+        d -= np.round(d / L) * L
+        say(f"  lanes {lanes}: max |dx| vs one lane {np.abs(d).max():.2e}")
+        assert np.abs(d).max() < 1e-11
+        assert force_rel_err(out[lanes][2], out[1][2]) < 1e-11
