@@ -208,6 +208,19 @@ int tdmd_eam_compute(const tdmd_eam_tables* t, const double* d_pos,
                      const int32_t* d_counts, int32_t natoms,
                      const tdmd_box* box, double* d_rho, void* d_fp,
                      double* d_energy, void* d_work, void* stream);
+/* No counterpart in the reference: tdmd_eam_compute for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same kernels; rho, F'(rho) and the
+ * fixed-order energy partials live in a library-owned device array of
+ * ~16.8 MB per device (held until the process ends), so at most
+ * 1,048,576 atoms (larger natoms: TDMD_E_INVALID, nothing launched). It
+ * calls no hipMalloc/hipFree. Calls on one device share that array: calls
+ * that overlap on one device (other streams, other threads) race. */
+int tdmd_eam_compute_noalloc(const tdmd_eam_tables* t, const double* d_pos,
+                             void* d_forces, const int32_t* d_types,
+                             const int32_t* d_neighbors, const int32_t* d_offsets,
+                             const int32_t* d_counts, int32_t natoms,
+                             const tdmd_box* box, double* d_energy, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Velocity Verlet (src/integrator/device_velocity_verlet.cuh:13-59).  */

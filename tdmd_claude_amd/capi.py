@@ -200,6 +200,8 @@ _PROTOS = {
     "tdmd_eam_tables_info": (C.c_int, [_VP, _PI32, _PD]),
     "tdmd_eam_compute": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _I32, C.POINTER(Box),
                                    _VP, _VP, _VP, _VP, _VP]),
+    "tdmd_eam_compute_noalloc": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _I32,
+                                           C.POINTER(Box), _VP, _VP]),
     "tdmd_half_kick": (C.c_int, [_I32, _VP, _VP, _VP, _VP, _I32, _D, _VP]),
     "tdmd_drift": (C.c_int, [_VP, _VP, _I32, _D, C.POINTER(Box), _VP]),
     "tdmd_fused_kick_drift": (C.c_int, [_I32, _I32, _VP, _VP, _VP, _VP, _VP, _I32, _D,

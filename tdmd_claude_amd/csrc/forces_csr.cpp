@@ -252,6 +252,42 @@ int eam_launch(const tdmd_eam_tables* t, const double* pos, void* forces, const 
   return TDMD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free EAM compute: the per-atom densities, F'(rho) and the
+// energy partials live in one file-scope device array instead of caller
+// buffers. Its device address is looked up once per device
+// (hipGetSymbolAddress) and kept; kernels get that address, never the
+// array's host-side name. Every call on a device shares the array: calls
+// that overlap on one device (several streams, several host threads) race.
+// The kernels store every element they later read (rho, F'(rho), one
+// partial per block), so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxAtoms = 1 << 20;
+constexpr size_t kScratchRho = 0;                                             // double[N]
+constexpr size_t kScratchFp = static_cast<size_t>(kNoAllocMaxAtoms) * 8;      // real[N]
+constexpr size_t kScratchPartials = static_cast<size_t>(kNoAllocMaxAtoms) * 16;  // double[grid]
+constexpr size_t kScratchBytes =
+    kScratchPartials + static_cast<size_t>((kNoAllocMaxAtoms + kBlock - 1) / kBlock + 64) * 8;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {};
+
+int scratch_for_current_device(unsigned char** out) {
+  int dev = 0;
+  TDMD_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(TDMD_E_INVALID, "eam_compute_noalloc: device index");
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    TDMD_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(TDMD_E_HIP, "eam_compute_noalloc: no device address for the scratch array");
+    g_scratch[dev] = p;
+  }
+  *out = static_cast<unsigned char*>(g_scratch[dev]);
+  return TDMD_OK;
+}
+
 }  // namespace
 
 }  // namespace tdmd_hip
@@ -318,6 +354,33 @@ int tdmd_eam_compute(const tdmd_eam_tables* t, const double* d_pos, void* d_forc
                              natoms, db, d_rho, d_fp, d_energy, d_work, as_stream(stream));
   return eam_launch<double>(t, d_pos, d_forces, d_types, d_neighbors, d_offsets, d_counts,
                             natoms, db, d_rho, d_fp, d_energy, d_work, as_stream(stream));
+}
+
+int tdmd_eam_compute_noalloc(const tdmd_eam_tables* t, const double* d_pos, void* d_forces,
+                             const int32_t* d_types, const int32_t* d_neighbors,
+                             const int32_t* d_offsets, const int32_t* d_counts, int32_t natoms,
+                             const tdmd_box* box, double* d_energy, void* stream) {
+  if (!t || !box || natoms < 0) return fail(TDMD_E_INVALID, "eam_compute_noalloc: bad arguments");
+  if (natoms > kNoAllocMaxAtoms)
+    return fail(TDMD_E_INVALID, "eam_compute_noalloc: " + std::to_string(natoms) +
+                                    " atoms exceed the scratch array's " +
+                                    std::to_string(kNoAllocMaxAtoms));
+  if (natoms == 0) return TDMD_OK;
+  if (!d_pos || !d_forces || !d_types || !d_offsets || !d_counts)
+    return fail(TDMD_E_INVALID, "eam_compute_noalloc: null pointer");
+  if (t->ntypes > 8) return fail(TDMD_E_INVALID, "eam_compute_noalloc: at most 8 element types");
+  unsigned char* base = nullptr;
+  const int rc = scratch_for_current_device(&base);
+  if (rc) return rc;
+  double* rho = reinterpret_cast<double*>(base + kScratchRho);
+  void* fp = base + kScratchFp;
+  void* work = base + kScratchPartials;
+  const DevBox db = make_devbox(*box);
+  if (t->precision == TDMD_MIXED)
+    return eam_launch<float>(t, d_pos, d_forces, d_types, d_neighbors, d_offsets, d_counts,
+                             natoms, db, rho, fp, d_energy, work, as_stream(stream));
+  return eam_launch<double>(t, d_pos, d_forces, d_types, d_neighbors, d_offsets, d_counts,
+                            natoms, db, rho, fp, d_energy, work, as_stream(stream));
 }
 
 }  // extern "C"
