@@ -60,14 +60,23 @@ def nvt_cfg(precision: str) -> FastPipelineConfig:
                               nhc_length=3, use_graph=True)
 
 
-def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[float, float]:
+def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int,
+              pass_clock: bool = False) -> tuple[float, float]:
     """Warm up, run one untimed window of the timed window's shape (so every
     hipGraph the timed window replays is captured and instantiated before
     it), then time exactly `steps` steps: host clock and HIP events on the
-    scheduler's stream, device synchronised on both sides."""
+    scheduler's stream, device synchronised on both sides. pass_clock: after
+    the timed window, one more window of the same shape in which the passes
+    time themselves on the device clock (its graphs captured in a
+    pre-capture window of their own), so the instrumentation's atomics stay
+    out of the timed window; read the result with sched.pass_times()."""
     sched.run_until(warmup)
-    sched.run_until(warmup + steps)  # graph pre-capture, untimed
-    start = warmup + steps
+    if pass_clock:  # capture the instrumented graphs
+        sched.pass_timing(True)
+        sched.run_until(sched.current_step() + steps)
+        sched.pass_timing(False)
+    sched.run_until(sched.current_step() + steps)  # graph pre-capture, untimed
+    start = sched.current_step()
     stream = sched.stream()
     e0, e1 = capi.C.c_void_p(), capi.C.c_void_p()
     capi.call("tdmd_event_create", capi.C.byref(e0))
@@ -83,6 +92,9 @@ def timed_run(sched: FastPipelineScheduler, warmup: int, steps: int) -> tuple[fl
     capi.call("tdmd_event_elapsed_ms", e0, e1, capi.C.byref(ms))
     capi.call("tdmd_event_destroy", e0)
     capi.call("tdmd_event_destroy", e1)
+    if pass_clock:  # the passes of the next window of the same shape
+        sched.pass_timing(True)
+        sched.run_until(start + 2 * steps)
     return wall, ms.value / 1000.0
 
 
@@ -159,6 +171,7 @@ def nve_drift(system: I.System, potential, precision: str, steps: int, every: in
     cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=precision,
                              rebuild_policy="displacement")
     sched = FastPipelineScheduler.for_system(system, potential, cfg)
+    sched.pass_timing(True)
     t, e = [], []
     for k in range(0, steps + 1, every):
         sched.run_until(k)
@@ -166,12 +179,13 @@ def nve_drift(system: I.System, potential, precision: str, steps: int, every: in
         t.append(k * 0.001)
         e.append((th["pe"] + th["ke"]) / system.natoms)
     rebuilds = sched.stats().rebuilds
+    force_ms = sched.pass_times()[1]
     sched.close()
     e = np.array(e)
     return dict(drift_ev_per_atom_per_ns=energy_slope(t, e),
                 max_dev_ev_per_atom=float(np.abs(e - e[0]).max()), steps=steps,
                 sample_every=every, discarded="first 25 %", precision=precision,
-                rebuilds=rebuilds)
+                rebuilds=rebuilds, force_pass_ms=force_ms)
 
 
 def bench_morse_c2(steps: int) -> dict:
@@ -247,7 +261,7 @@ def bench_replicas(steps: int) -> dict:
     three-element Johnson-form EAM (5000-point tables, cutoff 5.0 A), NVE,
     dt 1 fs, all replicas advanced together (one launch per pass), shape
     chosen by the autotuner: mixed and FP64 throughput."""
-    from tdmd_claude_amd.scheduler import BatchScheduler
+    from tdmd_claude_amd.scheduler import BatchScheduler, tune_cell_size
 
     systems, temps = I.alloy_replicas(64, 2024)
     tables = I.johnson_alloy_setfl()
@@ -260,9 +274,15 @@ def bench_replicas(steps: int) -> dict:
     for prec in ("mixed", "fp64"):
         cfg = FastPipelineConfig(dt=0.001, r_skin=1.0, precision=prec,
                                  rebuild_policy="displacement")
+        # tuner: the binning cell size on trial schedulers, then the kernel
+        # shape (lanes per row) in place
+        cells = tune_cell_size(systems, tables, cfg, candidates=(100, 125, 150))
+        cfg.cell_pct = cells["cell_pct"]
         b = BatchScheduler.for_systems(systems, tables, cfg)
         b.run_until(0)
         tuned = b.autotune(5)
+        tuned["cell_pct"] = cells["cell_pct"]
+        tuned["cell_ms_per_step"] = {k: round(v, 4) for k, v in cells["ms_per_step"].items()}
         wall, _ = timed_run(b, 50, steps)
         out[prec] = {"value": round(natoms * steps / wall / 1e6, 2),
                      "ms_per_step": round(1000 * wall / steps, 4),
@@ -571,16 +591,14 @@ def main() -> int:
         sched = FastPipelineScheduler.for_system(
             system, tables, FastPipelineConfig(**{**nvt_cfg("mixed").__dict__,
                                                   "mirror_fp64": args.mirror_fp64}))
-        wall, dev = timed_run(sched, args.warmup, args.steps)
-        log(f"profile run: {args.steps} steps in {wall:.3f} s")
+        wall, dev = timed_run(sched, args.warmup, args.steps, pass_clock=True)
+        ta, tb = sched.pass_times()
+        log(f"profile run: {args.steps} NVT steps in {wall:.3f} s; passes of the timed window: "
+            f"density {ta:.4f} ms, force {tb:.4f} ms")
+        pos_e, vel_e, _, _, _ = sched.download()
         sched.close()
-        state = thermal_state(system, tables)
-        for label, cfg in (("NVT", nvt_cfg("mixed")), ("NVE", FastPipelineConfig())):
-            cfg.mirror_fp64 = args.mirror_fp64
-            p = FastPipelineScheduler.for_system(state, tables, cfg)
-            ta, tb = p.time_passes(30)
-            log(f"{label} passes (thermalised state): density {ta:.4f} ms, force {tb:.4f} ms")
-            p.close()
+        state = system.copy()
+        state.pos, state.vel = pos_e, vel_e
         # a list rebuild every 5 steps: the fused rebuild's own cost in a trace
         reb = FastPipelineScheduler.for_system(
             state, tables, FastPipelineConfig(precision="mixed", rebuild_policy="fixed",
@@ -594,13 +612,19 @@ def main() -> int:
     cfg.mirror_fp64 = args.mirror_fp64
     sched = FastPipelineScheduler.for_system(system, tables, cfg)
     log("mixed: warmup, graph pre-capture window, timed window")
-    wall, dev = timed_run(sched, args.warmup, args.steps)
+    # the passes' mean durations from their own device clock, over the
+    # window of the same shape that follows the timed one
+    wall, dev = timed_run(sched, args.warmup, args.steps, pass_clock=True)
+    t_dens, t_force = sched.pass_times()
     th = sched.thermo()
     value = N * args.steps / wall / 1e6
     stats = sched.stats()
     log(f"mixed: {args.steps} steps, wall {wall:.4f} s (events {dev:.4f} s) -> {value:.1f} "
         f"Matom-steps/s; T {th['temp']:.1f} K, mean neighbours {th['mean_nbr']:.2f}, "
         f"rebuilds {stats.rebuilds}")
+    obs = sched.observables()
+    nbar, nnear = obs["mean_neighbors"], obs["mean_near_neighbors"]
+    halo = sched.halo_info()
     pos_m, vel_m, f_m, _, _ = sched.download()
     sched.close()
 
@@ -614,21 +638,7 @@ def main() -> int:
     s64.close()
     log(f"fp64: {v64:.1f} Matom-steps/s")
 
-    # ---- per-pass timing of the benchmark's NVT step on the thermalised state ----
-    state = thermal_state(system, tables)
-    ncfg = nvt_cfg("mixed")
-    ncfg.mirror_fp64 = args.mirror_fp64
-    nvt = FastPipelineScheduler.for_system(state, tables, ncfg)
-    nvt.run_until(0)
-    obs = nvt.observables()
-    nbar, nnear = obs["mean_neighbors"], obs["mean_near_neighbors"]
-    t_dens, t_force = nvt.time_passes(30)
-    halo = nvt.halo_info()
-    nvt.close()
-    nve = FastPipelineScheduler.for_system(
-        state, tables, FastPipelineConfig(precision="mixed", mirror_fp64=args.mirror_fp64))
-    _, t_force_nve = nve.time_passes(30)
-    nve.close()
+    # ---- the roofline of the timed window's force pass ----
     staged = halo["staged"] / halo["blocks"] if halo["blocks"] else 0.0
     halo_pa = halo["halo_atoms"] / N if halo["blocks"] else 0.0
     bpa = bytes_per_atom("mixed", args.mirror_fp64, nbar, staged, halo_pa, nnear)
@@ -636,19 +646,22 @@ def main() -> int:
     achieved = N * b_alg / (t_force * 1e-3) / 1e9
     traffic = measured_traffic(N, args.mirror_fp64, "force_nvt")
     flops = measured_flops(N, args.mirror_fp64, "force_nvt")
-    log(f"NVT passes: density {t_dens:.4f} ms, force {t_force:.4f} ms (NVE force "
-        f"{t_force_nve:.4f} ms); force pass {achieved:.0f} GB/s on {b_alg:.1f} B/atom")
-    # steps between rebuilds over the whole mixed run (warmup, pre-capture and
-    # timed windows; the first build is not a rebuild)
-    run_steps = args.warmup + 2 * args.steps
+    log(f"timed-window passes: density {t_dens:.4f} ms, force {t_force:.4f} ms; force pass "
+        f"{achieved:.0f} GB/s on {b_alg:.1f} B/atom")
+    # steps between rebuilds over the whole mixed run (warmup, two pre-capture
+    # windows, the timed and the pass-clock window; the first build is not a
+    # rebuild)
+    run_steps = args.warmup + 4 * args.steps
     n_rb = run_steps / (stats.rebuilds - 1) if stats.rebuilds > 1 else float(run_steps)
 
     # ---- NVE energy drift from the thermalised state, mixed and FP64 ----
+    state = thermal_state(system, tables)
     drift = {}
     for prec in ("mixed", "fp64"):
         drift[prec] = nve_drift(state, tables, prec, args.nve_steps, 500)
         log(f"NVE drift {prec}: {drift[prec]['drift_ev_per_atom_per_ns']:.3e} eV/atom/ns "
             f"over {args.nve_steps} steps")
+    t_force_nve = drift["mixed"]["force_pass_ms"]
 
     # ---- other workloads of the path (not the metric) ----
     extras = {}
@@ -719,7 +732,8 @@ def main() -> int:
             "mean_neighbours": round(nbar, 2),
             "rebuilds": stats.rebuilds,
             "kernel_launches_per_step": round(stats.kernel_launches / max(stats.ticks, 1), 2),
-            "untimed_precapture_steps": args.steps,
+            "untimed_precapture_steps": 2 * args.steps,
+            "pass_clock_window_steps": args.steps,
         },
         "events_s": round(dev, 5),
         "fp64": {k: round(v, 4) if isinstance(v, float) else v for k, v in fp64.items()},
@@ -733,6 +747,7 @@ def main() -> int:
             "bound": "hbm",
             "kernel": "k_eam_force_lds<float, MirrorFixed, 3, ...> (force + closing half kick "
                       "+ KE partials: the force pass of the timed NVT step)",
+            "window": "the timed window's steps, repeated with the pass clock on",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -744,6 +759,9 @@ def main() -> int:
                                    "int32 full-list row, F (the reference's layout)",
             "bytes_per_atom_own_layout": round(bpa["force_nvt"], 1),
             "avg_ms": round(t_force, 5),
+            "avg_ms_source": "the pass's own device-clock spans (first workgroup start to "
+                             "last workgroup finish), mean over the launches of the window of "
+                             "the timed window's shape that follows it",
             "density_pass_ms": round(t_dens, 5),
             "nve_force_pass_ms": round(t_force_nve, 5),
             "fp32_tflops": None if flops is None else round(flops["fp32_ops"] / (t_force * 1e9), 2),

@@ -311,10 +311,13 @@ typedef struct {
   int32_t disable_halo;   /* 1: no LDS halo staging in the single-element */
                           /* EAM passes (0: staged where records are 16 B)*/
   int32_t pair_lanes;     /* lanes per neighbour row in the force passes:  */
-                          /* 0 = by size (8 up to 131072 atoms, else 1),  */
+                          /* 0 = by size (8 up to 131072 atoms for Morse, */
+                          /* 65536 for EAM, else 1),                      */
                           /* 1, 4 or 8; EAM with more than one lane runs  */
                           /* the global-table passes (no LDS tables/halo) */
-  int32_t reserved[3];
+  int32_t cell_pct;       /* binning cells at least this percentage of     */
+                          /* r_list per axis: 0 (= 100) or 100..400       */
+  int32_t reserved[2];
 } tdmd_sched_config;
 
 void tdmd_sched_config_default(tdmd_sched_config* cfg);
@@ -400,14 +403,35 @@ int tdmd_sched_observables(tdmd_sched* s, tdmd_observables* out);
  * KE partials, the pass of the thermostatted step). Advances the state like
  * run_until. NVT is accepted for EAM schedulers only. */
 int tdmd_sched_time_passes(tdmd_sched* s, int32_t steps, float* ms_out);
-/* Autotuner (no reference counterpart): times `iters` density + force
- * passes (force-only epilogue, no change to the trajectory) for each
- * candidate kernel shape -- neighbour slots per pipelined group {8, 4} x
- * LDS spline window {device limit, 79 KiB (two workgroups per CU)} -- on
- * the uploaded state, keeps the fastest and returns it: out[0] =
- * pair_unroll, out[1] = lds_budget bytes (0 where the potential has no LDS
- * window to choose). Runs the initial force pass if needed. */
+/* In-window pass timing (no reference counterpart; measurement aid): with
+ * `on`, the density and force passes of every step (inside the captured
+ * hipGraphs too) time themselves on the device wall clock: each workgroup
+ * stamps its start, the last one to finish adds the launch's span (first
+ * workgroup start to last workgroup finish) to a running sum. Each call
+ * with `on` resets the sums. tdmd_sched_pass_times returns the mean span
+ * per launch since then: ms_out[0] = EAM density pass (Morse: 0),
+ * ms_out[1] = force pass. Graphs captured with timing on are kept apart
+ * from those without. (HIP events recorded inside a captured graph are not
+ * timing events on this runtime, hence the device clock.) */
+int tdmd_sched_pass_timing(tdmd_sched* s, int32_t on);
+int tdmd_sched_pass_times(tdmd_sched* s, float* ms_out);
+/* Autotuner (no reference counterpart): times `iters` force-path passes
+ * (EAM density + force, Morse force; force-only epilogue) for each
+ * candidate kernel shape on the uploaded state -- lanes per neighbour row
+ * {1, 4, 8}, and with one lane and single-element EAM tables the LDS
+ * passes with {8, 4} neighbour slots per pipelined group (x LDS spline
+ * window {device limit, 79 KiB} where no halo is staged) -- and keeps the
+ * fastest (the starting shape when within 1 % of it). The passes are then
+ * run once more with the chosen shape, so the state holds that shape's
+ * forces: a choice equal to the starting shape leaves the trajectory
+ * bitwise unchanged, another lane count changes the order of the row sums
+ * only. Returns out[0] = pair_unroll, out[1] = lds_budget bytes (0 off the
+ * LDS passes); tdmd_sched_kernel_shape reports the lanes. Runs the initial
+ * force pass if needed. */
 int tdmd_sched_autotune(tdmd_sched* s, int32_t iters, int32_t out[2]);
+/* The current kernel shape (no reference counterpart): out[0] = lanes per
+ * neighbour row, out[1] = pair_unroll, out[2] = lds_budget. */
+int tdmd_sched_kernel_shape(const tdmd_sched* s, int32_t out[3]);
 /* Halo staging (no reference counterpart; diagnostics): out[0] = blocks of
  * 1024 atoms, out[1] = blocks whose halo is staged in LDS, out[2] = largest
  * halo (atoms), out[3] = halo atoms summed over all blocks. All zero when

@@ -86,7 +86,8 @@ class SchedConfig(C.Structure):
         ("split_rebuild", C.c_int32),
         ("disable_halo", C.c_int32),
         ("pair_lanes", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("cell_pct", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -241,7 +242,10 @@ _PROTOS = {
     "tdmd_sched_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
     "tdmd_sched_halo_info": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
     "tdmd_sched_time_passes": (C.c_int, [_VP, _I32, C.POINTER(C.c_float)]),
+    "tdmd_sched_pass_timing": (C.c_int, [_VP, _I32]),
+    "tdmd_sched_pass_times": (C.c_int, [_VP, C.POINTER(C.c_float)]),
     "tdmd_sched_autotune": (C.c_int, [_VP, _I32, _PI32]),
+    "tdmd_sched_kernel_shape": (C.c_int, [_VP, _PI32]),
     "tdmd_pipe_create": (C.c_int, [C.POINTER(_VP), C.POINTER(Box), _I32, C.POINTER(MorseParams),
                                    C.POINTER(PipeConfig)]),
     "tdmd_pipe_destroy": (C.c_int, [_VP]),

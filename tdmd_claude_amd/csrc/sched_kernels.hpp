@@ -244,7 +244,37 @@ struct StepArgs {
   const BoxGeo* boxes;   // [nrep]
   const int32_t* rep;    // [N] replica of each slot (static: the sort keeps
                          // every replica in its own slot range)
+  // Pass clock (measurement aid, null when off): 4 words per pass kind
+  // (0 density, 1 force): earliest workgroup start of the running launch,
+  // workgroups finished, sum of launch spans, launches (device wall clock)
+  unsigned long long* pclock;
 };
+
+// Pass clock: every workgroup stamps the device wall clock at entry; the
+// last workgroup to finish a launch adds the span from the earliest stamp
+// to its own finish to the running total and re-arms the words for the
+// next launch. Device-scope atomics only (no fences: a release fence here
+// would write back the XCD's L2 once per workgroup); every thread of the
+// workgroup must reach pass_clock_end (it holds a workgroup barrier).
+__device__ __forceinline__ void pass_clock_begin(unsigned long long* pc) {
+  if (pc && threadIdx.x == 0)
+    __hip_atomic_fetch_min(pc, static_cast<unsigned long long>(wall_clock64()), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void pass_clock_end(unsigned long long* pc) {
+  if (!pc) return;
+  __syncthreads();
+  if (threadIdx.x == 0 &&
+      __hip_atomic_fetch_add(pc + 1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+          static_cast<unsigned long long>(gridDim.x) - 1) {
+    const unsigned long long now = static_cast<unsigned long long>(wall_clock64());
+    const unsigned long long t0 = __hip_atomic_load(pc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(pc + 2, now - t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(pc + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(pc, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(pc + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
 // Mirror parameters of the box atom i belongs to.
 template <class Mir>

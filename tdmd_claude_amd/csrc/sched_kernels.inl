@@ -423,6 +423,7 @@ k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
               const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
   static_assert(kLanes >= 1 && 64 % kLanes == 0, "an atom's lanes share a wavefront");
   using T = typename Mir::T;
+  pass_clock_begin(a.pclock);
   __shared__ SplineMeta sm[kMaxSplines];
   load_meta(sm, meta, 2 * a.ntypes);
   const int blk = xcd_block(gridDim.x);
@@ -462,6 +463,7 @@ k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
     e = static_cast<double>(F.f);
   }
   if (kEnergy) emit_partial(e, a.partials, blk);
+  pass_clock_end(a.pclock);
 }
 
 template <class R, class Mir, int kEpi, bool kEnergy, bool kMulti, int kLanes>
@@ -470,6 +472,8 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
             const typename Quad<R>::type* __restrict__ coeff, double rc_sq) {
   static_assert(kLanes >= 1 && 64 % kLanes == 0, "an atom's lanes share a wavefront");
   using T = typename Mir::T;
+  unsigned long long* const pclk = a.pclock ? a.pclock + 4 : nullptr;
+  pass_clock_begin(pclk);
   __shared__ SplineMeta sm[kMaxSplines];
   const int nt = a.ntypes;
   load_meta(sm, meta, 2 * nt + nt * (nt + 1) / 2);
@@ -532,6 +536,7 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
   if (kEnergy) emit_partial(vir, a.partials_w, blk);
+  pass_clock_end(pclk);
 }
 
 // ---------------------------------------------------------------------------
@@ -545,6 +550,8 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
   // kLanes > 1: kLanes lanes per atom, the grid kLanes times larger.
   static_assert(kLanes >= 1 && 64 % kLanes == 0, "an atom's lanes share a wavefront");
   using T = typename Mir::T;
+  unsigned long long* const pclk = a.pclock ? a.pclock + 4 : nullptr;
+  pass_clock_begin(pclk);
   const int blk = xcd_block(gridDim.x);
   const int g = blk * kBlock + threadIdx.x;
   const int i = g / kLanes;
@@ -593,6 +600,7 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
   }
   if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
   if (kEnergy) emit_partial(vir, a.partials_w, blk);
+  pass_clock_end(pclk);
 }
 
 // ---------------------------------------------------------------------------
@@ -809,6 +817,41 @@ __device__ __forceinline__ void copyback_atom(const StepArgs& a, const GridArgs&
   static_cast<typename Mir::T*>(a.M)[s] = Mir::encode(mp_of<Mir>(a, s), x, y, z, ty - 1);
 }
 
+// Candidate prefilter of the list search with the 32-bit fixed-point
+// records (MirrorFixed): r^2 in FP32 from the records' wrapping differences.
+// The records round each coordinate to one unit (at most L / 2^32, or
+// W / 2^30 on an open axis; cells are at most 1024 per axis, so a unit is
+// below 1e-6 of r_list) and FP32 adds ~1e-6 relative, so a candidate whose
+// FP32 r^2 lies outside [1 - kPreMargin, 1 + kPreMargin] times a bound is on
+// the same side of it as its exact FP64 r^2. Inside that band (a few
+// candidates per million atoms) the exact test decides.
+constexpr double kPreMargin = 1e-5;
+
+struct PreBounds {
+  float lo_list, hi_list, lo_near, hi_near;
+  float ux, uy, uz;
+};
+
+__device__ __forceinline__ PreBounds pre_bounds(const MirrorParams& mp, double r_list_sq,
+                                                double near_sq) {
+  PreBounds b;
+  b.lo_list = static_cast<float>(r_list_sq * (1.0 - kPreMargin));
+  b.hi_list = static_cast<float>(r_list_sq * (1.0 + kPreMargin));
+  b.lo_near = static_cast<float>(near_sq * (1.0 - kPreMargin));
+  b.hi_near = static_cast<float>(near_sq * (1.0 + kPreMargin));
+  b.ux = static_cast<float>(mp.unit[0]);
+  b.uy = static_cast<float>(mp.unit[1]);
+  b.uz = static_cast<float>(mp.unit[2]);
+  return b;
+}
+
+__device__ __forceinline__ float pre_r2(const int4& mj, const int4& mi, const PreBounds& b) {
+  const float dx = static_cast<float>(MirrorFixed::wdiff(mj.x, mi.x)) * b.ux;
+  const float dy = static_cast<float>(MirrorFixed::wdiff(mj.y, mi.y)) * b.uy;
+  const float dz = static_cast<float>(MirrorFixed::wdiff(mj.z, mi.z)) * b.uz;
+  return dx * dx + dy * dy + dz * dz;
+}
+
 // ELL row of atom i over the sorted atoms; returns the true count. Inclusion
 // exactly as the CPU list (neighbor_list.cpp:58-62): d = x_j - x_i in FP64,
 // single minimum image, unfused r2 < r_list^2. kCount: only count.
@@ -839,6 +882,12 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
       g.around + 27 * static_cast<size_t>(
                           g.rank_of[cg.off + (c[2] * cg.n[1] + c[1]) * cg.n[0] + c[0]]);
   const size_t row = static_cast<size_t>(around - g.around);
+  int4 mi_rec = make_int4(0, 0, 0, 0);
+  PreBounds pb{};
+  if constexpr (Mir::kFixed) {
+    mi_rec = static_cast<const int4*>(a.M)[i];
+    pb = pre_bounds(mp, r_list_sq, near_sq);
+  }
   // One copy of the search per image mode, so the loop body has no branch
   // between a group's coordinate loads and they issue together.
   auto search = [&](auto images) {
@@ -859,6 +908,56 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
         sx = (img & 3) == 1 ? mp.L[0] : -mp.L[0];
         sy = ((img >> 2) & 3) == 1 ? mp.L[1] : -mp.L[1];
         sz = ((img >> 4) & 3) == 1 ? mp.L[2] : -mp.L[2];
+      }
+      auto exact_r2 = [&](int j) {
+        double ddx = a.px[j] - xi, ddy = a.py[j] - yi, ddz = a.pz[j] - zi;
+        if constexpr (kImg) {
+          ddx = ix ? ddx + sx : ddx;
+          ddy = iy ? ddy + sy : ddy;
+          ddz = iz ? ddz + sz : ddz;
+        } else {
+          ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
+          ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
+          ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
+        }
+        return r2_unfused(ddx, ddy, ddz);
+      };
+      if constexpr (Mir::kFixed) {
+        // one 16-byte record per candidate, prefiltered in FP32; the exact
+        // test where the prefilter is too close to call
+        const int4* __restrict__ M = static_cast<const int4*>(a.M);
+        for (int j0 = start; j0 < end; j0 += 4) {
+          int4 mq[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) mq[q] = M[min(j0 + q, end - 1)];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = j0 + q;
+            if (j < end && j != i) {
+              const float r2f = pre_r2(mq[q], mi_rec, pb);
+              bool in, near;
+              if ((r2f >= pb.lo_list && r2f <= pb.hi_list) ||
+                  (r2f >= pb.lo_near && r2f <= pb.hi_near)) {
+                const double r2 = exact_r2(j);
+                in = r2 < r_list_sq;
+                near = r2 < near_sq;
+              } else {
+                in = r2f < pb.lo_list;
+                near = r2f < pb.lo_near;
+              }
+              if (in) {
+                if (!kCount && nn + nf < cap) {
+                  if (near)
+                    nbr[ell(i, nn++, cap)] = j;
+                  else
+                    nbr[ell(i, cap - 1 - nf++, cap)] = j;
+                }
+                ++count;
+              }
+            }
+          }
+        }
+        continue;
       }
       // candidates in groups of four: the group's coordinates are loaded
       // together, then tested and appended in index order
@@ -966,6 +1065,12 @@ __device__ __forceinline__ int nlist_atom_group(const StepArgs& a, const GridArg
     my_end = my_start + g.ccount[r];
     my_img = g.around_img[row + l];
   }
+  int4 mi_rec = make_int4(0, 0, 0, 0);
+  PreBounds pb{};
+  if constexpr (Mir::kFixed) {
+    mi_rec = static_cast<const int4*>(a.M)[i];
+    pb = pre_bounds(mp, r_list_sq, near_sq);
+  }
   int count = 0, nn = 0, nf = 0;
   for (int t = 0; t < 27; ++t) {
     const int start = __shfl(my_start, sh + t);
@@ -981,21 +1086,41 @@ __device__ __forceinline__ int nlist_atom_group(const StepArgs& a, const GridArg
       const int j = j0 + l;
       const bool valid = j < end;
       const int jj = valid ? j : i;
-      double ddx = a.px[jj] - xi, ddy = a.py[jj] - yi, ddz = a.pz[jj] - zi;
-      if (images) {
-        ddx = ix ? ddx + sx : ddx;
-        ddy = iy ? ddy + sy : ddy;
-        ddz = iz ? ddz + sz : ddz;
+      auto exact_r2 = [&]() {
+        double ddx = a.px[jj] - xi, ddy = a.py[jj] - yi, ddz = a.pz[jj] - zi;
+        if (images) {
+          ddx = ix ? ddx + sx : ddx;
+          ddy = iy ? ddy + sy : ddy;
+          ddz = iz ? ddz + sz : ddz;
+        } else {
+          ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
+          ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
+          ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
+        }
+        return r2_unfused(ddx, ddy, ddz);
+      };
+      bool in_list, in_near;
+      if constexpr (Mir::kFixed) {
+        // the record prefilter of nlist_atom, the exact test in its band
+        const float r2f = pre_r2(static_cast<const int4*>(a.M)[jj], mi_rec, pb);
+        if ((r2f >= pb.lo_list && r2f <= pb.hi_list) ||
+            (r2f >= pb.lo_near && r2f <= pb.hi_near)) {
+          const double r2 = exact_r2();
+          in_list = r2 < r_list_sq;
+          in_near = r2 < near_sq;
+        } else {
+          in_list = r2f < pb.lo_list;
+          in_near = r2f < pb.lo_near;
+        }
       } else {
-        ddx = min_image(ddx, mp.L[0], mp.hL[0], mp.periodic[0]);
-        ddy = min_image(ddy, mp.L[1], mp.hL[1], mp.periodic[1]);
-        ddz = min_image(ddz, mp.L[2], mp.hL[2], mp.periodic[2]);
+        const double r2 = exact_r2();
+        in_list = r2 < r_list_sq;
+        in_near = r2 < near_sq;
       }
-      const double r2 = r2_unfused(ddx, ddy, ddz);
-      const bool hit = valid && j != i && r2 < r_list_sq;
+      const bool hit = valid && j != i && in_list;
       const uint32_t hg = static_cast<uint32_t>(__ballot(hit) >> sh);
       const bool kept = hit && count + __popc(hg & lt) < cap;
-      const bool near = kept && r2 < near_sq;
+      const bool near = kept && in_near;
       const uint32_t kg = static_cast<uint32_t>(__ballot(kept) >> sh);
       const uint32_t ng = static_cast<uint32_t>(__ballot(near) >> sh);
       if (kept) {

@@ -183,6 +183,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   using Q = typename Quad<R>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Q* tab = reinterpret_cast<Q*>(smem);
+  pass_clock_begin(a.pclock);
   const SplineMeta mF = meta[0], mrho = meta[1];
   const SplineMetaF mrho_f = meta_f32(mrho);
   for (int q = threadIdx.x; q < w.nwin; q += kBigBlock) tab[q] = coeff[mrho.offset + w.k_lo + q];
@@ -245,6 +246,7 @@ k_eam_density_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   if (kEnergy) emit_partial<kBigBlock>(e, a.partials, blk);
   __syncthreads();  // the next block's halo overwrites this one
   }
+  pass_clock_end(a.pclock);
 }
 
 template <class R, class Mir, int kEpi, bool kEnergy, bool kWin, int kU, bool kStage,
@@ -260,6 +262,8 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   Q* tA = reinterpret_cast<Q*>(smem);              // {phi b, phi c, phi d, rho b}
   V2* tB = reinterpret_cast<V2*>(tA + w.nwin);     // {rho c, rho d}
   R* tE = reinterpret_cast<R*>(tB + w.nwin);       // phi a (energy only)
+  unsigned long long* const pclk = a.pclock ? a.pclock + 4 : nullptr;
+  pass_clock_begin(pclk);
   const SplineMeta mrho = meta[1], mphi = meta[2];
   for (int q = threadIdx.x; q < w.nwin; q += kBigBlock) {
     const Q cp = coeff[mphi.offset + w.k_lo + q];
@@ -355,6 +359,7 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   if (kEnergy) emit_partial<kBigBlock>(wsum, a.partials_w, blk);
   __syncthreads();  // the next block's halo overwrites this one
   }
+  pass_clock_end(pclk);
 }
 
 }  // namespace sched

@@ -39,12 +39,17 @@ namespace {
 
 using namespace sched;
 
-// Rows of systems up to kSplitMax atoms are split over kSplitLanes lanes in
-// the force passes (Morse, and EAM through the global-table passes): one
-// lane per atom would give 32,000 atoms 125 workgroups on 256 CUs, each lane
-// walking ~80 slots in turn; the LDS EAM passes would run 32 workgroups.
+// Rows of small systems are split over kSplitLanes lanes in the force
+// passes (Morse, and EAM through the global-table passes): one lane per atom
+// would give 32,000 atoms 125 workgroups on 256 CUs, each lane walking ~80
+// slots in turn; the LDS EAM passes would run 32 workgroups. Thresholds from
+// the measured pass times (mixed, 8 lanes against one): Morse 32,000 atoms
+// 10.6 against 19.9 us, 108,000 26.3 against 28.9, 256,000 57.9 against
+// 51.1; EAM density + force 32,000 22.7 against 53.7 us (LDS passes),
+// 108,000 74.6 against 59.3.
 constexpr int kSplitLanes = 8;
-constexpr int kSplitMax = 131072;
+constexpr int kSplitMax = 131072;     // Morse
+constexpr int kEamSplitMax = 65536;   // EAM
 // The fused rebuild's list search takes 32 lanes per atom up to this size.
 constexpr int kNlistSplitMax = 131072;
 
@@ -91,6 +96,8 @@ struct EngineBase {
   // Time the candidate kernel shapes on the current state and keep the
   // fastest; out = {pair_unroll, lds_budget bytes (0: not applicable)}.
   virtual int tune(int iters, int32_t* out) = 0;
+  // {lanes per row, neighbour slots per pipelined group, LDS window budget}
+  virtual void shape(int32_t* out) const = 0;
   // Bracket the next force-path launch (EAM density / force, Morse
   // kick-drift / force) with events stamped by the dispatch itself.
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -109,6 +116,7 @@ struct tdmd_sched {
   int nblocks = 0;   // 256-thread grid (multiple of 8)
   int nb_force = 0;  // grid of the EAM/Morse passes = block partials they emit
   int lanes = 1;  // lanes per atom of the force passes (Morse, global-table EAM)
+  bool stage_ok = false;  // halo buffers exist: the staged LDS passes can be chosen
   int nb_rebuild = 0;  // fused rebuild grid (all resident at once); 0: separate kernels
   tdmd_hip::sched::GridBarrier* d_bar = nullptr;
   tdmd_sched_config cfg{};
@@ -134,6 +142,10 @@ struct tdmd_sched {
   bool rec_swapped = false;
   int current_step = 0, steps_since = 0;
   int64_t ticks = 0, launches = 0, host_rebuilds = 0, graph_launches = 0;
+  // In-window pass timing: the passes' own device clock (StepArgs::pclock)
+  bool pass_timing = false;
+  unsigned long long* d_pclock = nullptr;  // 8 words: density, force
+  int clock_khz = 0;                       // device wall clock rate
   std::map<std::string, hipGraphExec_t> graphs;
   std::map<std::string, hipGraph_t> graph_defs;
   std::map<std::string, int64_t> graph_kernels;
@@ -181,6 +193,8 @@ struct Engine final : EngineBase {
   // Halo staging (k_halo + the kStage passes) for 16-byte gather records.
   static constexpr bool kCanStage = sizeof(typename Mir::T) == 16;
   bool stage = false;
+  bool lds_ok = false;  // single-element tables on one r grid: the LDS passes can run
+  int nb_lds = 0;       // 1024-atom blocks of the LDS passes (multiple of 8)
   bool iso = false;  // fixed-point mirror with one unit on all axes (every box)
   int ncu = 0;
   size_t lds_cu = 160 * 1024;  // LDS of one CU
@@ -248,14 +262,27 @@ struct Engine final : EngineBase {
     // holds of the largest pass (at most two of 1024 threads), per CU
     const size_t per_wg = std::max(b_dens[0], b_force[0]) + stage_bytes() + 1024;
     const int per_cu = std::max(1, std::min(2, static_cast<int>(lds_cu / per_wg)));
-    lds_grid = std::max(8, std::min(S->nb_force, (std::max(ncu, 8) / 8) * 8 * per_cu));
+    lds_grid = std::max(8, std::min(nb_lds, (std::max(ncu, 8) / 8) * 8 * per_cu));
+  }
+
+  // Kernel path for `lanes` lanes per row: one lane with single-element
+  // tables runs the LDS passes (halo-staged where the halo buffers exist);
+  // otherwise the global-table passes with rows split over `lanes` lanes.
+  // The fused rebuild builds halos only while the staged passes are chosen.
+  void set_path(int lanes) {
+    S->lanes = lanes;
+    lds = lanes == 1 && lds_ok;
+    stage = lds && S->stage_ok;
+    S->a.stage_cap = stage ? kStageCap : 0;
+    S->nb_force = lds ? nb_lds : S->nblocks * lanes;
   }
 
   int init() override {
     // rows split over kSplitLanes lanes when one lane per atom would leave
     // the chip mostly idle (partials: one per block of the larger grid)
-    S->lanes = S->cfg.pair_lanes > 0 ? S->cfg.pair_lanes
-                                     : (S->natoms <= kSplitMax ? kSplitLanes : 1);
+    S->lanes = S->cfg.pair_lanes > 0
+                   ? S->cfg.pair_lanes
+                   : (S->natoms <= (S->kind == 1 ? kEamSplitMax : kSplitMax) ? kSplitLanes : 1);
     S->nb_force = S->nblocks * S->lanes;
     S->nb_rebuild = 0;
     if (!S->cfg.split_rebuild) {
@@ -277,7 +304,6 @@ struct Engine final : EngineBase {
       S->g.nl_split = split && S->nb_rebuild > 0 ? 1 : 0;
     }
     if (S->kind != 1 || S->tables->ntypes != 1) return TDMD_OK;
-    if (S->lanes > 1) return TDMD_OK;  // split rows: the global-table passes
     // rho_0 and phi_00 must share the r grid (setfl: one Nr, dr).
     SplineMeta m[3];
     TDMD_HIP_TRY(hipMemcpy(m, S->tables->d_meta, sizeof(m), hipMemcpyDeviceToHost));
@@ -300,8 +326,8 @@ struct Engine final : EngineBase {
             hipSuccess &&
         lcu > 0)
       lds_cu = static_cast<size_t>(lcu);
-    lds = true;
-    S->nb_force = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
+    lds_ok = true;
+    nb_lds = round_up((S->natoms + kBigBlock - 1) / kBigBlock, 8);
     if constexpr (kCanStage) {
       iso = true;
       auto same_unit = [](const MirrorParams& p) {
@@ -315,7 +341,7 @@ struct Engine final : EngineBase {
       if (iso && (rc = allow_shapes<false, true>())) return rc;
       if (!S->cfg.disable_halo && lds_limit >= static_cast<int>(kStageCap) * 16 + 16384) {
         if ((rc = (iso ? allow_shapes<true, true>() : allow_shapes<true, false>()))) return rc;
-        const size_t nbf = static_cast<size_t>(S->nb_force);
+        const size_t nbf = static_cast<size_t>(nb_lds);
         void* p = nullptr;
         const size_t sizes[5] = {nbf * kHaloMax * 4, nbf * 4, nbf * 4, 4,
                                  static_cast<size_t>(round_up(S->natoms, 64)) * 4};
@@ -332,10 +358,10 @@ struct Engine final : EngineBase {
         S->a.cntc = static_cast<int32_t*>(ptrs[4]);
         TDMD_HIP_TRY(hipMemset(ptrs[2], 0xff, sizes[2]));  // no halo yet: not staged
         TDMD_HIP_TRY(hipMemset(d_halo_done, 0, 4));
-        S->a.stage_cap = kStageCap;
-        stage = true;
+        S->stage_ok = true;
       }
     }
+    set_path(S->lanes);
     configure(S->cfg.pair_unroll == 0 ? 8 : S->cfg.pair_unroll, S->cfg.lds_budget);
     return TDMD_OK;
   }
@@ -349,53 +375,92 @@ struct Engine final : EngineBase {
     ++S->launches;
   }
 
+  void shape(int32_t* out) const override {
+    out[0] = S->lanes;
+    out[1] = lds ? unroll : kPairUnroll;
+    out[2] = lds ? budget : 0;
+  }
+
+  // The force-path passes of one step with the force-only epilogue: they
+  // rewrite M2 and F from the current positions and change nothing else.
+  void probe_passes(hipStream_t s) {
+    if (S->kind == 1) density<false>(s);
+    force<kForceOnly, false>(s);
+  }
+
   int tune(int iters, int32_t* out) override {
-    out[0] = lds ? unroll : kPairUnroll;
-    out[1] = lds ? budget : 0;
-    if (!lds || S->kind != 1) return TDMD_OK;  // nothing to choose
+    // Candidates: lanes per row {1, 4, 8}; with one lane and single-element
+    // tables the LDS passes, in two pipelined group sizes (and, without
+    // halo staging, two spline-window budgets).
+    struct Cand {
+      int lanes, u, b;
+    };
+    std::vector<Cand> cands;
+    if (lds_ok) {
+      cands.push_back({1, 8, lds_limit});
+      cands.push_back({1, 4, lds_limit});
+      if (!S->stage_ok) {
+        cands.push_back({1, 8, 80 * 1024 - 1024});
+        cands.push_back({1, 4, 80 * 1024 - 1024});
+      }
+    } else {
+      cands.push_back({1, 0, 0});
+    }
+    cands.push_back({4, 0, 0});
+    cands.push_back({8, 0, 0});
     hipStream_t s = S->stream;
     hipEvent_t e0, e1;
     TDMD_HIP_TRY(hipEventCreate(&e0));
     TDMD_HIP_TRY(hipEventCreate(&e1));
-    struct Cand {
-      int u, b;
-    };
-    const Cand cands[] = {{8, lds_limit}, {4, lds_limit}, {8, 80 * 1024 - 1024},
-                          {4, 80 * 1024 - 1024}};
-    // with halo staging the halo already takes half the LDS: unroll only
-    const int ncand = stage ? 2 : 4;
-    float best = 1e30f;
-    Cand pick = cands[0];
-    for (int ci = 0; ci < ncand; ++ci) {
-      const Cand& c = cands[ci];
-      configure(c.u, c.b);
-      // density + force with the force-only epilogue: both passes rewrite
-      // M2 and F with the values they already hold, so timing is free of
-      // side effects on the trajectory
-      for (int k = 0; k < 2; ++k) {
-        density<false>(s);
-        force<kForceOnly, false>(s);
-      }
+    const Cand start{S->lanes, lds ? unroll : 0, lds ? budget : 0};
+    std::vector<float> times;
+    for (const Cand& c : cands) {
+      if (int rc = apply_shape(c.lanes, c.u, c.b, s)) return rc;
+      for (int k = 0; k < 2; ++k) probe_passes(s);
       TDMD_HIP_TRY(hipEventRecord(e0, s));
-      for (int k = 0; k < iters; ++k) {
-        density<false>(s);
-        force<kForceOnly, false>(s);
-      }
+      for (int k = 0; k < iters; ++k) probe_passes(s);
       TDMD_HIP_TRY(hipEventRecord(e1, s));
       TDMD_HIP_TRY(hipEventSynchronize(e1));
       TDMD_LAUNCH_CHECK();
       float ms = 0.0f;
       TDMD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        pick = c;
-      }
+      times.push_back(ms);
     }
-    configure(pick.u, pick.b);
+    // the fastest shape; the starting shape when it is within 1 % of it
+    size_t k_best = 0;
+    for (size_t k = 1; k < cands.size(); ++k)
+      if (times[k] < times[k_best]) k_best = k;
+    Cand pick = cands[k_best];
+    for (size_t k = 0; k < cands.size(); ++k) {
+      const Cand& c = cands[k];
+      const bool is_start =
+          c.lanes == start.lanes && (c.lanes != 1 || !lds_ok || (c.u == start.u && c.b == start.b));
+      if (is_start && times[k] <= 1.01f * times[k_best]) pick = c;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    out[0] = unroll;
-    out[1] = budget;
+    if (int rc = apply_shape(pick.lanes, pick.u, pick.b, s)) return rc;
+    // M2 and F as the chosen shape computes them, so the trajectory goes on
+    // exactly as it would have with that shape (bitwise unchanged when the
+    // choice is the starting shape)
+    probe_passes(s);
+    TDMD_LAUNCH_CHECK();
+    out[0] = lds ? unroll : kPairUnroll;
+    out[1] = lds ? budget : 0;
+    return TDMD_OK;
+  }
+
+  // Switch the kernel shape (tuner): path, group size, window budget. A
+  // switch to the staged passes rebuilds the halos of the current rows.
+  int apply_shape(int lanes, int u, int b, hipStream_t s) {
+    const bool was_staged = stage;
+    set_path(lanes);
+    if (lds) configure(u, b);
+    if (stage && !was_staged) {
+      TDMD_HIP_TRY(hipMemsetAsync(S->a.flags + 4, 0xff, 4, s));  // halos due
+      halo(s);
+      TDMD_LAUNCH_CHECK();
+    }
     return TDMD_OK;
   }
 
@@ -817,7 +882,8 @@ int sched_alloc(tdmd_sched* S) {
 }
 
 // Mirror parameters and binning grid of one box.
-int setup_box(const tdmd_box& box, double r_list, MirrorParams* mp, int n[3], double inv_cs[3]) {
+int setup_box(const tdmd_box& box, double r_list, double cell_min, MirrorParams* mp, int n[3],
+              double inv_cs[3]) {
   for (int d = 0; d < 3; ++d) {
     mp->lo[d] = box.lo[d];
     mp->L[d] = box.hi[d] - box.lo[d];
@@ -834,11 +900,11 @@ int setup_box(const tdmd_box& box, double r_list, MirrorParams* mp, int n[3], do
     mp->q[d] = std::ldexp(1.0, e) / W;
     mp->unit[d] = std::ldexp(W, -e);
   }
-  // Binning grid: cells no smaller than r_list (with a relative margin so
-  // rounding in the bin computation can never push a neighbour out of the
-  // 27-cell stencil), at least 3 per axis (cell_list.cpp:22-27).
+  // Binning grid: cells no smaller than cell_min >= r_list (with a relative
+  // margin so rounding in the bin computation can never push a neighbour
+  // out of the 27-cell stencil), at least 3 per axis (cell_list.cpp:22-27).
   for (int d = 0; d < 3; ++d) {
-    n[d] = std::max(3, static_cast<int>(std::floor(mp->L[d] / (r_list * (1.0 + 1e-9)))));
+    n[d] = std::max(3, static_cast<int>(std::floor(mp->L[d] / (cell_min * (1.0 + 1e-9)))));
     inv_cs[d] = static_cast<double>(n[d]) / mp->L[d];
     if (n[d] > 1024) return fail(TDMD_E_INVALID, "scheduler: more than 1024 cells per axis");
   }
@@ -871,6 +937,8 @@ int sched_setup(tdmd_sched* S, const std::vector<tdmd_box>& boxes, const std::ve
   if (cfg.lds_budget < 0) return fail(TDMD_E_INVALID, "scheduler: lds_budget must be >= 0");
   if (cfg.pair_lanes != 0 && cfg.pair_lanes != 1 && cfg.pair_lanes != 4 && cfg.pair_lanes != 8)
     return fail(TDMD_E_INVALID, "scheduler: pair_lanes must be 0 (by size), 1, 4 or 8");
+  if (cfg.cell_pct != 0 && (cfg.cell_pct < 100 || cfg.cell_pct > 400))
+    return fail(TDMD_E_INVALID, "scheduler: cell_pct must be 0 or 100..400");
   S->box = boxes[0];
   S->natoms = natoms;
   S->cfg = cfg;
@@ -878,7 +946,8 @@ int sched_setup(tdmd_sched* S, const std::vector<tdmd_box>& boxes, const std::ve
   S->nblocks = round_up(grid_for(natoms), 8);
   S->r_list = S->cutoff + cfg.r_skin;
   GridArgs& g = S->g;
-  int rc = setup_box(boxes[0], S->r_list, &S->a.mp, g.n, g.inv_cs);
+  const double cell_min = S->r_list * (cfg.cell_pct > 0 ? cfg.cell_pct : 100) / 100.0;
+  int rc = setup_box(boxes[0], S->r_list, cell_min, &S->a.mp, g.n, g.inv_cs);
   if (rc) return rc;
   g.ncells = g.n[0] * g.n[1] * g.n[2];
   if (multibox) {
@@ -888,7 +957,7 @@ int sched_setup(tdmd_sched* S, const std::vector<tdmd_box>& boxes, const std::ve
     for (size_t b = 0; b < boxes.size(); ++b) {
       BoxGeo& bg = S->h_boxes[b];
       std::memset(&bg, 0, sizeof(bg));
-      if ((rc = setup_box(boxes[b], S->r_list, &bg.mp, bg.n, bg.inv_cs))) return rc;
+      if ((rc = setup_box(boxes[b], S->r_list, cell_min, &bg.mp, bg.n, bg.inv_cs))) return rc;
       bg.cell_off = cells;
       cells += bg.n[0] * bg.n[1] * bg.n[2];
       S->rep_off.push_back(S->rep_off.back() + counts[b]);
@@ -996,7 +1065,7 @@ int first_build(tdmd_sched* S) {
     void* p = nullptr;
     if ((rc = tdmd_malloc(&p, slots * 4))) return rc;
     S->a.nbr = static_cast<int32_t*>(p);
-    if (S->a.stage_cap > 0) {  // 16-bit rows of the staged passes, full and cut
+    if (S->stage_ok) {  // 16-bit rows of the staged passes, full and cut
       if ((rc = tdmd_malloc(&p, slots * 2))) return rc;
       S->a.nbr16 = static_cast<uint16_t*>(p);
       if ((rc = tdmd_malloc(&p, slots * 2))) return rc;
@@ -1016,26 +1085,29 @@ int first_build(tdmd_sched* S) {
   return TDMD_OK;
 }
 
+int launch_ops(tdmd_sched* S, const std::string& ops, hipStream_t s) {
+  for (char op : ops) {
+    const int rc = S->eng->launch(op, s);
+    if (rc) return rc;
+  }
+  return TDMD_OK;
+}
+
 // Enqueue a sequence of ops, as one cached graph when enabled.
 int run_ops(tdmd_sched* S, const std::string& ops) {
   if (ops.empty()) return TDMD_OK;
   hipStream_t s = S->stream;
-  if (!S->cfg.use_graph) {
-    for (char op : ops) {
-      const int rc = S->eng->launch(op, s);
-      if (rc) return rc;
-    }
-    return TDMD_OK;
-  }
-  // a graph holds the record buffers of the state it was captured in
-  const std::string key = ops + (S->rec_swapped ? "/1" : "/0");
+  if (!S->cfg.use_graph) return launch_ops(S, ops, s);
+  // a graph holds the record buffers of the state it was captured in (and
+  // the pass clock, when pass timing is on)
+  const std::string key = ops + (S->rec_swapped ? "/1" : "/0") + (S->pass_timing ? "t" : "");
   auto it = S->graphs.find(key);
   if (it == S->graphs.end()) {
     const int64_t saved = S->launches;
     const bool swapped0 = S->rec_swapped;
     TDMD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (char op : ops) {
-      const int rc = S->eng->launch(op, s);
+    {
+      const int rc = launch_ops(S, ops, s);
       if (rc) {
         hipGraph_t dummy;
         (void)hipStreamEndCapture(s, &dummy);
@@ -1296,7 +1368,14 @@ int tdmd_sched_autotune(tdmd_sched* S, int32_t iters, int32_t out[2]) {
   S->graph_swaps.clear();
   S->cfg.pair_unroll = out[0];
   S->cfg.lds_budget = out[1];
+  S->cfg.pair_lanes = S->lanes;
   return check_flags(S);
+}
+
+int tdmd_sched_kernel_shape(const tdmd_sched* S, int32_t out[3]) {
+  if (!S || !out || !S->eng) return fail(TDMD_E_INVALID, "sched_kernel_shape: null");
+  S->eng->shape(out);
+  return TDMD_OK;
 }
 
 int tdmd_sched_destroy(tdmd_sched* s) {
@@ -1501,6 +1580,40 @@ int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
   S->steps_since += steps;
   S->ticks += steps;
   return check_flags(S);
+}
+
+int tdmd_sched_pass_timing(tdmd_sched* S, int32_t on) {
+  if (!S) return fail(TDMD_E_INVALID, "sched_pass_timing: null");
+  if (on && !S->d_pclock) {
+    int dev = 0;
+    TDMD_HIP_TRY(hipGetDevice(&dev));
+    TDMD_HIP_TRY(hipDeviceGetAttribute(&S->clock_khz, hipDeviceAttributeWallClockRate, dev));
+    if (S->clock_khz <= 0) return fail(TDMD_E_HIP, "sched_pass_timing: no device wall clock");
+    int rc = dalloc(S, &S->d_pclock, 8);
+    if (rc) return rc;
+  }
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  if (on) {
+    const unsigned long long init[8] = {~0ull, 0, 0, 0, ~0ull, 0, 0, 0};
+    TDMD_HIP_TRY(hipMemcpy(S->d_pclock, init, sizeof(init), hipMemcpyHostToDevice));
+  }
+  S->pass_timing = on != 0;
+  S->a.pclock = on ? S->d_pclock : nullptr;
+  return TDMD_OK;
+}
+
+int tdmd_sched_pass_times(tdmd_sched* S, float* ms_out) {
+  if (!S || !ms_out) return fail(TDMD_E_INVALID, "sched_pass_times: null");
+  if (!S->pass_timing) return fail(TDMD_E_STATE, "sched_pass_times: pass timing is off");
+  TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+  unsigned long long w[8];
+  TDMD_HIP_TRY(hipMemcpy(w, S->d_pclock, sizeof(w), hipMemcpyDeviceToHost));
+  if (w[7] == 0) return fail(TDMD_E_STATE, "sched_pass_times: no force pass ran since timing on");
+  for (int p = 0; p < 2; ++p)
+    ms_out[p] = w[4 * p + 3] ? static_cast<float>(static_cast<double>(w[4 * p + 2]) /
+                                                  static_cast<double>(w[4 * p + 3]) / S->clock_khz)
+                             : 0.0f;
+  return TDMD_OK;
 }
 
 int tdmd_sched_halo_info(tdmd_sched* S, int64_t out[4]) {

@@ -44,6 +44,7 @@ class FastPipelineConfig:
     split_rebuild: bool = False  # rebuild as separate kernels instead of one fused launch
     disable_halo: bool = False  # no LDS halo staging in the single-element EAM passes
     pair_lanes: int = 0  # lanes per neighbour row: 0 = by size, 1, 4 or 8
+    cell_pct: int = 0  # binning cells >= this % of r_list per axis (0 = 100)
 
     def to_c(self) -> capi.SchedConfig:
         c = capi.SchedConfig()
@@ -65,6 +66,7 @@ class FastPipelineConfig:
         c.split_rebuild = int(self.split_rebuild)
         c.disable_halo = int(self.disable_halo)
         c.pair_lanes = self.pair_lanes
+        c.cell_pct = self.cell_pct
         return c
 
 
@@ -184,12 +186,33 @@ class FastPipelineScheduler:
         capi.call("tdmd_sched_time_passes", self._h, int(steps), out)
         return float(out[0]), float(out[1])
 
+    def pass_timing(self, on: bool = True) -> None:
+        """Time the density and force passes of every step on the device
+        clock (inside the captured graphs too); each call with on=True
+        resets the sums."""
+        capi.call("tdmd_sched_pass_timing", self._h, int(on))
+
+    def pass_times(self) -> tuple[float, float]:
+        """Mean ms per launch of the density and force passes since pass
+        timing was (re)started."""
+        out = (C.c_float * 2)()
+        capi.call("tdmd_sched_pass_times", self._h, out)
+        return float(out[0]), float(out[1])
+
     def autotune(self, iters: int = 10) -> dict:
-        """Pick the fastest kernel shape on the uploaded state (no change to
-        the trajectory); returns {"pair_unroll", "lds_budget"}."""
+        """Pick the fastest kernel shape on the uploaded state (lanes per row,
+        and for the LDS passes the group size and window); returns
+        {"pair_lanes", "pair_unroll", "lds_budget"}."""
         out = (C.c_int32 * 2)()
         capi.call("tdmd_sched_autotune", self._h, int(iters), out)
-        return {"pair_unroll": int(out[0]), "lds_budget": int(out[1])}
+        return self.kernel_shape()
+
+    def kernel_shape(self) -> dict:
+        """The kernel shape in use: {"pair_lanes", "pair_unroll", "lds_budget"}."""
+        out = (C.c_int32 * 3)()
+        capi.call("tdmd_sched_kernel_shape", self._h, out)
+        return {"pair_lanes": int(out[0]), "pair_unroll": int(out[1]),
+                "lds_budget": int(out[2])}
 
     def halo_info(self) -> dict:
         """LDS halo staging: 1024-atom blocks, blocks staged, largest and total halo."""
@@ -272,3 +295,35 @@ class BatchScheduler(FastPipelineScheduler):
             a, b = self.offsets[r], self.offsets[r + 1]
             out.append((pos[a:b], vel[a:b], f[a:b], types[a:b], ids[a:b]))
         return out
+
+
+def tune_cell_size(systems, potential, cfg: FastPipelineConfig | None = None,
+                   candidates=(100, 125, 150), steps: int = 30, rebuild_every: int = 10) -> dict:
+    """Cell-size tuner: times a trial scheduler on the given state for each
+    candidate binning cell size (percent of r_list) -- `steps` steps with a
+    list rebuild every `rebuild_every` steps, after a warm-up window of the
+    same shape -- and returns the fastest as {"cell_pct", "ms_per_step":
+    {pct: ms}}. The cell size changes the spatial sort (atoms are ordered by
+    cell) and so the order of the row sums, never which pairs interact: runs
+    with different cell sizes agree to rounding. `systems` is one System or
+    a list of replicas (BatchScheduler)."""
+    import time
+
+    base = cfg or FastPipelineConfig()
+    times = {}
+    for pct in candidates:
+        trial = dataclasses.replace(base, cell_pct=int(pct), rebuild_policy="fixed",
+                                    rebuild_every=rebuild_every)
+        if isinstance(systems, list):
+            sched = BatchScheduler.for_systems(systems, potential, trial)
+        else:
+            sched = FastPipelineScheduler.for_system(systems, potential, trial)
+        sched.run_until(steps)  # initial build, graph capture
+        sched.run_until(2 * steps)
+        capi.call("tdmd_device_synchronize")
+        t0 = time.perf_counter()
+        sched.run_until(3 * steps)
+        times[int(pct)] = 1000.0 * (time.perf_counter() - t0) / steps
+        sched.close()
+    best = min(times, key=times.get)
+    return {"cell_pct": best, "ms_per_step": times}

@@ -280,25 +280,58 @@ def test_launch_accounting():
     assert 5.0 <= per_step <= 5.2
 
 
-def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged():
-    s = fcc(6)
-    say("autotune")
-    cfg = FastPipelineConfig(precision="mixed", pair_lanes=1)  # the LDS passes' shapes
-    ref = FastPipelineScheduler.for_system(s, potential("johnson"), cfg)
+@pytest.mark.parametrize("pot,lanes", [("johnson", 1), ("johnson", 0), ("morse", 0),
+                                       ("alloy", 0)])
+def test_autotune_picks_a_shape_and_leaves_the_trajectory_unchanged(pot, lanes):
+    # the tuner times lanes per row {1, 4, 8} (and the LDS passes' shapes),
+    # keeps the fastest and reruns the passes with it: the same shape as the
+    # start leaves the trajectory bitwise unchanged, another lane count only
+    # the order of the row sums (FP64: rounding)
+    if pot == "alloy":
+        from golden.make_golden import alloy_system, alloy_tables
+
+        s, params = alloy_system("alloy3"), alloy_tables("alloy3")
+    else:
+        s, params = fcc(6), potential(pot)
+    say("autotune", pot, lanes)
+    cfg = FastPipelineConfig(precision="fp64", pair_lanes=lanes)
+    ref = FastPipelineScheduler.for_system(s, params, cfg)
+    start = ref.kernel_shape()
+    ref.run_until(10)  # the same two windows as the tuned run
     ref.run_until(20)
     p0, v0, _, _, _ = ref.download()
-    tuned = FastPipelineScheduler.for_system(s, potential("johnson"), cfg)
+    tuned = FastPipelineScheduler.for_system(s, params, cfg)
     tuned.run_until(10)
     choice = tuned.autotune(3)
-    assert choice["pair_unroll"] in (4, 8) and choice["lds_budget"] > 0
+    say("  start", start, "choice", choice)
+    assert choice == tuned.kernel_shape()
+    assert choice["pair_lanes"] in (1, 4, 8)
+    if pot == "johnson" and choice["pair_lanes"] == 1:
+        assert choice["pair_unroll"] in (4, 8) and choice["lds_budget"] > 0
     tuned.run_until(20)
     p1, v1, _, _, _ = tuned.download()
-    # same trajectory up to the kernel shape's summation order (bitwise when
-    # the choice equals the default shape)
     L = s.hi - s.lo
     d = p1 - p0
     d -= np.round(d / L) * L
-    assert np.abs(d).max() < 1e-9
+    if choice == start:
+        assert np.array_equal(p1, p0) and np.array_equal(v1, v0)
+    else:
+        assert np.abs(d).max() < 1e-11
+
+
+def test_kernel_shape_by_size():
+    # 8 lanes per row up to 65,536 atoms for EAM (global-table passes) and
+    # 131,072 for Morse; one lane (EAM: the LDS passes) above
+    s = fcc(5)
+    for params, want in ((potential("johnson"), 8), (MORSE_CU, 8)):
+        sched = FastPipelineScheduler.for_system(s, params, FastPipelineConfig())
+        assert sched.kernel_shape()["pair_lanes"] == want
+        sched.close()
+    sched = FastPipelineScheduler.for_system(s, potential("johnson"),
+                                             FastPipelineConfig(pair_lanes=1))
+    shape = sched.kernel_shape()
+    assert shape["pair_lanes"] == 1 and shape["lds_budget"] > 0
+    sched.close()
 
 
 @pytest.mark.parametrize("pot", ["johnson", "morse13"])
@@ -459,3 +492,54 @@ def test_row_split_is_deterministic_and_agrees_with_one_lane(pot):
         say(f"  lanes {lanes}: max |dx| vs one lane {np.abs(d).max():.2e}")
         assert np.abs(d).max() < 1e-11
         assert force_rel_err(out[lanes][2], out[1][2]) < 1e-11
+
+
+@pytest.mark.parametrize("pot", ["johnson", "morse"])
+def test_pass_timing_leaves_the_trajectory_unchanged(pot):
+    # events recorded around the passes inside the captured graphs time the
+    # passes of the window itself and change nothing in it
+    s = fcc(6)
+    say("pass timing", pot)
+    out = []
+    for on in (False, True):
+        cfg = FastPipelineConfig(precision="mixed", pair_lanes=1,
+                                 t_target=300.0 if pot == "johnson" else 0.0)
+        sched = FastPipelineScheduler.for_system(s, potential(pot), cfg)
+        sched.pass_timing(on)
+        sched.run_until(37)
+        out.append(sched.download()[:3])
+        if on:
+            dens, force = sched.pass_times()
+            say(f"  density {dens:.4f} ms, force {force:.4f} ms")
+            assert 0.0 < force < 50.0
+            assert (0.0 < dens < 50.0) if pot == "johnson" else dens == 0.0
+        sched.close()
+    for k in range(3):
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+def test_cell_size_tuner_agrees_to_rounding():
+    # the cell size changes the spatial sort and so the order of the row
+    # sums, never the pairs: the tuner's pick and the default give the same
+    # FP64 trajectory to rounding
+    from tdmd_claude_amd.scheduler import tune_cell_size
+
+    s = fcc(9)  # 9 cells of 3.6 A per axis at 100 %, 6 at 150 %
+    say("cell-size tuner")
+    res = tune_cell_size(s, potential("johnson"), FastPipelineConfig(precision="fp64"),
+                         candidates=(100, 150), steps=10)
+    say("  ", res)
+    assert res["cell_pct"] in (100, 150) and set(res["ms_per_step"]) == {100, 150}
+    out = []
+    for pct in (100, 150):
+        sched = FastPipelineScheduler.for_system(
+            s, potential("johnson"), FastPipelineConfig(precision="fp64", cell_pct=pct))
+        sched.run_until(40)
+        out.append(sched.download()[:3])
+        sched.close()
+    L = s.hi - s.lo
+    d = out[1][0] - out[0][0]
+    d -= np.round(d / L) * L
+    say(f"  max |dx| 150 % vs 100 %: {np.abs(d).max():.2e}")
+    assert np.abs(d).max() < 1e-10
+    assert force_rel_err(out[1][2], out[0][2]) < 1e-10
