@@ -543,3 +543,48 @@ def test_cell_size_tuner_agrees_to_rounding():
     say(f"  max |dx| 150 % vs 100 %: {np.abs(d).max():.2e}")
     assert np.abs(d).max() < 1e-10
     assert force_rel_err(out[1][2], out[0][2]) < 1e-10
+
+
+@pytest.mark.parametrize("mirror_fp64", [False, True])
+def test_list_counts_match_oracle_at_the_cutoffs(mirror_fp64):
+    # The list search decides most candidates from the 32-bit fixed-point
+    # records in FP32, with the exact FP64 test inside a 1e-5 relative band
+    # around r_list^2 and rc^2. Pairs placed 1e-7 from r_list and from rc
+    # (inside the band) and 3e-5 from them (outside it): the rows' total,
+    # near total and longest row must be the oracle's exact lists, at the
+    # first build (one lane per atom) and after a fused rebuild (32 lanes).
+    rng = np.random.Generator(np.random.PCG64(17))
+    L, rc, skin = 26.0, 6.0, 1.0
+    r_list = rc + skin
+    base = rng.uniform(0.0, L, size=(300, 3))
+    extra = []
+    for d in (r_list * (1 - 1e-7), r_list * (1 + 1e-7), rc * (1 - 1e-7), rc * (1 + 1e-7),
+              r_list * (1 - 3e-5), r_list * (1 + 3e-5), rc * (1 - 3e-5), rc * (1 + 3e-5)):
+        for _ in range(6):
+            p = rng.uniform(0.0, L, size=3)
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            extra += [p, np.mod(p + d * u, L)]
+    pos = np.concatenate([base, np.array(extra)])
+    s = I.System(lo=np.zeros(3), hi=np.full(3, L), pos=pos, vel=np.zeros_like(pos),
+                 types=np.ones(len(pos), dtype=np.int32),
+                 ids=np.arange(1, len(pos) + 1, dtype=np.int32), masses=np.array([1.0, 63.546]))
+    say("list counts at the cutoffs, mirror_fp64", mirror_fp64, s.natoms)
+    full = O.HalfList(s.pos, s.lo, s.hi, rc, skin).counts
+    near = O.HalfList(s.pos, s.lo, s.hi, rc, 0.0).counts
+    sched = FastPipelineScheduler.for_system(
+        s, MORSE_CU[:3] + (rc,), FastPipelineConfig(precision="mixed", mirror_fp64=mirror_fp64,
+                                                    rebuild_every=1, r_skin=skin, dt=1e-9))
+    for step in (0, 1):
+        sched.run_until(step)
+        if step == 1:  # the fused rebuild, after a step too short to move the pairs off
+            pos1 = sched.download()[0]
+            full = O.HalfList(pos1, s.lo, s.hi, rc, skin).counts
+            near = O.HalfList(pos1, s.lo, s.hi, rc, 0.0).counts
+        obs = sched.observables()
+        n_full = 2 * int(np.sum(full))
+        say(f"  step {step}: rows {obs['mean_neighbors'] * s.natoms:.0f} vs {n_full}, near "
+            f"{obs['mean_near_neighbors'] * s.natoms:.0f} vs {2 * int(np.sum(near))}")
+        assert round(obs["mean_neighbors"] * s.natoms) == n_full
+        assert round(obs["mean_near_neighbors"] * s.natoms) == 2 * int(np.sum(near))
+    sched.close()
