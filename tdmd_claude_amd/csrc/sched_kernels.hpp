@@ -248,6 +248,13 @@ struct StepArgs {
   // (0 density, 1 force): earliest workgroup start of the running launch,
   // workgroups finished, sum of launch spans, launches (device wall clock)
   unsigned long long* pclock;
+  // Several elements: per ordered element pair (ti, tj) and r interval, the
+  // derivative coefficients the force pass needs, interleaved -- {phi_titj
+  // b c d, rho_tj b | rho_tj c d, rho_ti b c | rho_ti d, 0, 0, 0} -- three
+  // quads per interval instead of one quad from each of three tables (null:
+  // the tables do not share one r grid)
+  const void* pairtab;
+  int pair_nint;  // r intervals per element pair
 };
 
 // Pass clock: every workgroup stamps the device wall clock at entry; the

@@ -496,16 +496,30 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
         const PairR<R> pr = pair_r<R>(r2);
         const int tj = kMulti ? a.types[j] - 1 : 0;
         const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
-        R dphi;
-        if (kEnergy) {
-          const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, pr.r);
-          pe += 0.5 * static_cast<double>(ph.f);
-          dphi = ph.df;
+        R dphi, drho_j, drho_i;
+        if (kMulti && !kEnergy && a.pairtab) {
+          // one interval (the three splines share the r grid), three
+          // adjacent quads; the same expressions as spline_deriv
+          using Q = typename Quad<R>::type;
+          R dx;
+          const int idx = spline_locate<R>(sm[2 * nt + pidx], pr.r, &dx);
+          const Q* e = static_cast<const Q*>(a.pairtab) +
+                       (static_cast<size_t>(ti * nt + tj) * a.pair_nint + idx) * 3;
+          const Q q0 = e[0], q1 = e[1], q2 = e[2];
+          dphi = q0.x + dx * (static_cast<R>(2) * q0.y + dx * static_cast<R>(3) * q0.z);
+          drho_j = q0.w + dx * (static_cast<R>(2) * q1.x + dx * static_cast<R>(3) * q1.y);
+          drho_i = q1.z + dx * (static_cast<R>(2) * q1.w + dx * static_cast<R>(3) * q2.x);
         } else {
-          dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, pr.r);
+          if (kEnergy) {
+            const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, pr.r);
+            pe += 0.5 * static_cast<double>(ph.f);
+            dphi = ph.df;
+          } else {
+            dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, pr.r);
+          }
+          drho_j = spline_deriv<R>(sm[nt + tj], coeff, pr.r);
+          drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
         }
-        const R drho_j = spline_deriv<R>(sm[nt + tj], coeff, pr.r);
-        const R drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
         const R fpj = Mir::template w_real<R>(mj);
         const double fpair = static_cast<double>(-(dphi + fpi * drho_j + fpj * drho_i) * pr.rinv);
         if (kEnergy) vir += 0.5 * fpair * r2;

@@ -265,6 +265,48 @@ struct Engine final : EngineBase {
     lds_grid = std::max(8, std::min(nb_lds, (std::max(ncu, 8) / 8) * 8 * per_cu));
   }
 
+  // Interleaved derivative table of the several-element force pass (see
+  // StepArgs::pairtab), from the tables' FP64 coefficients in the kernels'
+  // precision: the same values the per-spline quads hold.
+  int build_pair_table() {
+    const tdmd_eam_tables* t = S->tables;
+    const int nt = t->ntypes;
+    SplineMeta m[kMaxSplines];
+    const int ns = 2 * nt + nt * (nt + 1) / 2;
+    TDMD_HIP_TRY(hipMemcpy(m, t->d_meta, ns * sizeof(SplineMeta), hipMemcpyDeviceToHost));
+    for (int k = nt + 1; k < ns; ++k)  // rho and phi splines on one r grid
+      if (m[k].n != m[nt].n || m[k].dr != m[nt].dr || m[k].rmin != m[nt].rmin) return TDMD_OK;
+    const int nint = m[nt].n - 1;
+    std::vector<Q> h(static_cast<size_t>(nt) * nt * nint * 3);
+    auto co = [&](const std::vector<double>& v, int spline, int k) {
+      return static_cast<R>(v[static_cast<size_t>(t->offsets[static_cast<size_t>(spline)] + k)]);
+    };
+    for (int ti = 0; ti < nt; ++ti)
+      for (int tj = 0; tj < nt; ++tj) {
+        const int sp = 2 * nt + phi_index(ti, tj, nt), sj = nt + tj, si = nt + ti;
+        for (int k = 0; k < nint; ++k) {
+          Q* e = &h[(static_cast<size_t>(ti * nt + tj) * nint + k) * 3];
+          e[0].x = co(t->b, sp, k);
+          e[0].y = co(t->c, sp, k);
+          e[0].z = co(t->d, sp, k);
+          e[0].w = co(t->b, sj, k);
+          e[1].x = co(t->c, sj, k);
+          e[1].y = co(t->d, sj, k);
+          e[1].z = co(t->b, si, k);
+          e[1].w = co(t->c, si, k);
+          e[2].x = co(t->d, si, k);
+          e[2].y = e[2].z = e[2].w = static_cast<R>(0);
+        }
+      }
+    void* p = nullptr;
+    if (int rc = tdmd_malloc(&p, h.size() * sizeof(Q))) return rc;
+    S->allocs.push_back(p);
+    TDMD_HIP_TRY(hipMemcpy(p, h.data(), h.size() * sizeof(Q), hipMemcpyHostToDevice));
+    S->a.pairtab = p;
+    S->a.pair_nint = nint;
+    return TDMD_OK;
+  }
+
   // Kernel path for `lanes` lanes per row: one lane with single-element
   // tables runs the LDS passes (halo-staged where the halo buffers exist);
   // otherwise the global-table passes with rows split over `lanes` lanes.
@@ -303,6 +345,7 @@ struct Engine final : EngineBase {
       if (per_cu < 1) S->nb_rebuild = 0;  // cannot guarantee residency: separate kernels
       S->g.nl_split = split && S->nb_rebuild > 0 ? 1 : 0;
     }
+    if (S->kind == 1 && S->tables->ntypes > 1) return build_pair_table();
     if (S->kind != 1 || S->tables->ntypes != 1) return TDMD_OK;
     // rho_0 and phi_00 must share the r grid (setfl: one Nr, dr).
     SplineMeta m[3];
