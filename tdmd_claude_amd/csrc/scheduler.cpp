@@ -336,8 +336,14 @@ struct Engine final : EngineBase {
           &per_cu, reinterpret_cast<const void*>(&k_rebuild_fused<R, Mir>), kRebuildBlock, 0));
       // at most two workgroups per CU: a margin under the occupancy limit.
       // Small systems search each atom's cells with 32 lanes (one lane per
-      // atom leaves the search a chain of dependent loads on a few CUs).
-      const bool split = S->natoms <= kNlistSplitMax;
+      // atom leaves the search a chain of dependent loads on a few CUs) --
+      // under the fixed-interval policy. Under the displacement policy the
+      // launch is a no-op check on nearly every step (a 32,000-atom Morse
+      // system rebuilds once in 10,000 steps), and a grid sized for 32
+      // lanes per atom makes every check dispatch 16 times more workgroups:
+      // there the search takes one lane per atom.
+      const bool split = S->natoms <= kNlistSplitMax &&
+                         S->cfg.rebuild_policy != TDMD_REBUILD_DISPLACEMENT;
       const int64_t lanes = static_cast<int64_t>(S->natoms) * (split ? 32 : 1);
       const int need = static_cast<int>(
           std::max<int64_t>((lanes + kRebuildBlock - 1) / kRebuildBlock, 1));
