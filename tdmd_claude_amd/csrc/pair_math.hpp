@@ -8,9 +8,10 @@
 //     reference casts the FP64 difference to float first,
 //     device_morse.cu:69-74).
 //   * spline coefficients, the cubic in the interval offset dx, the
-//     potential functions and fpair are computed in R; dx itself is formed
-//     in FP64 (r - knot) and then rounded, so the interval position keeps
-//     full precision.
+//     potential functions and fpair are computed in R; dx itself keeps the
+//     precision of FP64 interval arithmetic: formed in FP64 (r - knot) and
+//     then rounded, or, in the mixed EAM force passes, from a float seed and
+//     its FP64 residual (pair_t_f32, below).
 //   * force and density sums are FP64.
 // For R = float, r comes from a float square root refined by one FP64
 // Newton step (error ~1e-16 relative), avoiding the slow FP64 sqrt.
@@ -57,7 +58,8 @@ __device__ __forceinline__ PairR<float> pair_r<float>(double r2) {
 // offset error of ~4e-7 A (float r, float interval coordinate) moves rho_i
 // by ~1e-7 relative, far inside the force tolerance, and saves the FP64
 // Newton step and FP64 interval arithmetic on every pair. The force pass
-// keeps the FP64 r, since its pair terms scale the FP64 difference vector.
+// keeps the offset's full precision (pair_t_f32), since its pair terms
+// scale the FP64 difference vector.
 struct SplineMetaF {
   float inv_dr;
   float off;  // rmin / dr
@@ -79,6 +81,67 @@ __device__ __forceinline__ int locate_f32(const SplineMetaF& m, double r2, float
   idx = idx > m.last ? m.last : idx;
   *dx = (u - static_cast<float>(idx)) * m.dr;
   return idx;
+}
+
+// Force pairs under mixed precision: 1/r, the interval and the offset t in
+// FP32, with t as accurate as FP64 interval arithmetic would give it.
+// One hardware reciprocal square root y of float(r^2) gives 1/r and the
+// seed r0 = r^2 y (~2 ulp). The seed's error delta = (r^2 - r0^2) / (2 r0)
+// comes from the FP64 residual (one Newton step, carried as a float beside
+// r0 rather than added in FP64). The interval is floor(r0 / dr); the offset
+// t = r0 - k dr + delta is formed in FP32 with dr = dr_hi + dr_lo
+// (dr_hi = float(dr)): fma(-k, dr_hi, r0) rounds once, on a number below
+// ~dr, so t keeps ~1e-10 A where float r0 - k dr alone would lose ~3e-7 A.
+// A pair within ~1e-6 A of a knot may take the neighbouring interval, with
+// t slightly outside [0, dr): the C1 FD-Hermite cubics agree there to
+// rounding. Every r spline of the tables starts at r = 0 and shares one
+// grid (setfl), which this relies on.
+struct ForceLocF {
+  float inv_dr;
+  float dr_hi, dr_lo;
+  float last;  // n - 2
+};
+
+__device__ __forceinline__ ForceLocF force_loc_f32(const SplineMeta& m) {
+  const float hi = static_cast<float>(m.dr);
+  return ForceLocF{static_cast<float>(m.inv_dr), hi,
+                   static_cast<float>(m.dr - static_cast<double>(hi)),
+                   static_cast<float>(m.n - 2)};
+}
+
+struct PairT {
+  float rinv;
+  float t;
+  int idx;
+  bool below;  // the interval lies below `lo` (idx and t are then not set)
+};
+
+// lo: the first interval the caller can read (a table window); the
+// interval is clamped to [lo, n - 2] in float (one v_med3_f32).
+__device__ __forceinline__ PairT pair_t_f32(const ForceLocF& m, double r2, float lo = 0.0f) {
+  const float f = static_cast<float>(r2);
+  const float y = __builtin_amdgcn_rsqf(f);
+  const float r0 = f * y;
+  const double e = fma(-static_cast<double>(r0), static_cast<double>(r0), r2);
+  const float delta = static_cast<float>(e) * (0.5f * y);
+  const float u = r0 * m.inv_dr;
+  const float kf = floorf(__builtin_amdgcn_fmed3f(u, lo, m.last));
+  PairT p;
+  p.t = fmaf(-kf, m.dr_lo, fmaf(-kf, m.dr_hi, r0)) + delta;
+  p.rinv = y;
+  p.idx = static_cast<int>(kf);
+  p.below = u < lo;
+  return p;
+}
+
+// Derivative of a + t (b + t (c + t d)) from b, 2c and 3d: the mixed force
+// pass's form (the tables it reads hold 2c and 3d, or it scales c and d by
+// 2 and 3 in float, which gives the same values).
+__device__ __forceinline__ float cubic_deriv_s(float b, float c2, float d3, float t) {
+  return fmaf(t, fmaf(t, d3, c2), b);
+}
+__device__ __forceinline__ float cubic_deriv_f(const float4& q, float t) {
+  return cubic_deriv_s(q.y, 2.0f * q.z, 3.0f * q.w, t);
 }
 
 template <class R>

@@ -251,8 +251,9 @@ struct StepArgs {
   // Several elements: per ordered element pair (ti, tj) and r interval, the
   // derivative coefficients the force pass needs, interleaved -- {phi_titj
   // b c d, rho_tj b | rho_tj c d, rho_ti b c | rho_ti d, 0, 0, 0} -- three
-  // quads per interval instead of one quad from each of three tables (null:
-  // the tables do not share one r grid)
+  // quads per interval instead of one quad from each of three tables; in
+  // mixed precision 2c and 3d in place of c and d (null: the tables do not
+  // share one r grid)
   const void* pairtab;
   int pair_nint;  // r intervals per element pair
 };

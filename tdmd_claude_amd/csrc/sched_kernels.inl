@@ -482,6 +482,8 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
   const int i = g / kLanes;
   const int part = g % kLanes;
   const T* __restrict__ M2 = static_cast<const T*>(a.M2);
+  using Q = typename Quad<R>::type;
+  constexpr bool kF32 = std::is_same_v<R, float>;
   double red = 0.0, vir = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
   if (i < a.natoms) {
     const T mi = M2[i];
@@ -489,39 +491,63 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
     const R fpi = Mir::template w_real<R>(mi);
     const int ti = kMulti ? a.types[i] - 1 : 0;
     const int n = a.cnt[i];
+    const ForceLocF floc = force_loc_f32(sm[nt]);  // every r spline's grid
     auto pair = [&](int j, const T& mj, bool in_row) {
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
-        const PairR<R> pr = pair_r<R>(r2);
         const int tj = kMulti ? a.types[j] - 1 : 0;
         const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
-        R dphi, drho_j, drho_i;
-        if (kMulti && !kEnergy && a.pairtab) {
-          // one interval (the three splines share the r grid), three
-          // adjacent quads; the same expressions as spline_deriv
-          using Q = typename Quad<R>::type;
-          R dx;
-          const int idx = spline_locate<R>(sm[2 * nt + pidx], pr.r, &dx);
-          const Q* e = static_cast<const Q*>(a.pairtab) +
-                       (static_cast<size_t>(ti * nt + tj) * a.pair_nint + idx) * 3;
-          const Q q0 = e[0], q1 = e[1], q2 = e[2];
-          dphi = q0.x + dx * (static_cast<R>(2) * q0.y + dx * static_cast<R>(3) * q0.z);
-          drho_j = q0.w + dx * (static_cast<R>(2) * q1.x + dx * static_cast<R>(3) * q1.y);
-          drho_i = q1.z + dx * (static_cast<R>(2) * q1.w + dx * static_cast<R>(3) * q2.x);
-        } else {
-          if (kEnergy) {
-            const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, pr.r);
-            pe += 0.5 * static_cast<double>(ph.f);
-            dphi = ph.df;
+        R dphi, drho_j, drho_i, rinv;
+        if constexpr (kF32) {
+          // mixed: pair_t_f32's interval and offset, derivatives from b,
+          // 2c, 3d (the pair table holds 2c and 3d)
+          const PairT p = pair_t_f32(floc, r2);
+          rinv = p.rinv;
+          if (kMulti && !kEnergy && a.pairtab) {
+            const Q* e = static_cast<const Q*>(a.pairtab) +
+                         (static_cast<size_t>(ti * nt + tj) * a.pair_nint + p.idx) * 3;
+            const Q q0 = e[0], q1 = e[1], q2 = e[2];
+            dphi = cubic_deriv_s(q0.x, q0.y, q0.z, p.t);
+            drho_j = cubic_deriv_s(q0.w, q1.x, q1.y, p.t);
+            drho_i = cubic_deriv_s(q1.z, q1.w, q2.x, p.t);
           } else {
-            dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, pr.r);
+            const Q cphi = coeff[sm[2 * nt + pidx].offset + p.idx];
+            if (kEnergy)
+              pe += 0.5 * static_cast<double>(
+                              cphi.x + p.t * (cphi.y + p.t * (cphi.z + p.t * cphi.w)));
+            dphi = cubic_deriv_f(cphi, p.t);
+            drho_j = cubic_deriv_f(coeff[sm[nt + tj].offset + p.idx], p.t);
+            drho_i = kMulti ? cubic_deriv_f(coeff[sm[nt + ti].offset + p.idx], p.t) : drho_j;
           }
-          drho_j = spline_deriv<R>(sm[nt + tj], coeff, pr.r);
-          drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
+        } else {
+          const PairR<R> pr = pair_r<R>(r2);
+          rinv = pr.rinv;
+          if (kMulti && !kEnergy && a.pairtab) {
+            // one interval (the three splines share the r grid), three
+            // adjacent quads; the same expressions as spline_deriv
+            R dx;
+            const int idx = spline_locate<R>(sm[2 * nt + pidx], pr.r, &dx);
+            const Q* e = static_cast<const Q*>(a.pairtab) +
+                         (static_cast<size_t>(ti * nt + tj) * a.pair_nint + idx) * 3;
+            const Q q0 = e[0], q1 = e[1], q2 = e[2];
+            dphi = q0.x + dx * (static_cast<R>(2) * q0.y + dx * static_cast<R>(3) * q0.z);
+            drho_j = q0.w + dx * (static_cast<R>(2) * q1.x + dx * static_cast<R>(3) * q1.y);
+            drho_i = q1.z + dx * (static_cast<R>(2) * q1.w + dx * static_cast<R>(3) * q2.x);
+          } else {
+            if (kEnergy) {
+              const SplineVal<R> ph = spline_both<R>(sm[2 * nt + pidx], coeff, pr.r);
+              pe += 0.5 * static_cast<double>(ph.f);
+              dphi = ph.df;
+            } else {
+              dphi = spline_deriv<R>(sm[2 * nt + pidx], coeff, pr.r);
+            }
+            drho_j = spline_deriv<R>(sm[nt + tj], coeff, pr.r);
+            drho_i = kMulti ? spline_deriv<R>(sm[nt + ti], coeff, pr.r) : drho_j;
+          }
         }
         const R fpj = Mir::template w_real<R>(mj);
-        const double fpair = static_cast<double>(-(dphi + fpi * drho_j + fpj * drho_i) * pr.rinv);
+        const double fpair = static_cast<double>(-(dphi + fpi * drho_j + fpj * drho_i) * rinv);
         if (kEnergy) vir += 0.5 * fpair * r2;
         fx = fma(fpair, dx, fx);
         fy = fma(fpair, dy, fy);

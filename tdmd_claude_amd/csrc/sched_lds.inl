@@ -26,7 +26,8 @@
 // (stage_cap records) leaves of the LDS budget.
 //
 // Force window entry: {phi b, phi c, phi d, rho b} + {rho c, rho d}
-// (+ phi a when the energy is wanted); density window entry: rho {a,b,c,d}.
+// (+ phi a when the energy is wanted; in the mixed force-only window 2c and
+// 3d in place of c and d); density window entry: rho {a,b,c,d}.
 // For one element phi_00, rho_0 share the r grid (setfl: one Nr, dr).
 #pragma once
 
@@ -262,24 +263,38 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
   Q* tA = reinterpret_cast<Q*>(smem);              // {phi b, phi c, phi d, rho b}
   V2* tB = reinterpret_cast<V2*>(tA + w.nwin);     // {rho c, rho d}
   R* tE = reinterpret_cast<R*>(tB + w.nwin);       // phi a (energy only)
+  // mixed: pairs located by pair_t_f32, derivatives from b, 2c, 3d; the
+  // force-only window holds 2c and 3d (the energy window c and d, scaled
+  // at use: the same values)
+  constexpr bool kF32 = std::is_same_v<R, float>;
+  constexpr bool kPre = kF32 && !kEnergy;
   unsigned long long* const pclk = a.pclock ? a.pclock + 4 : nullptr;
   pass_clock_begin(pclk);
   const SplineMeta mrho = meta[1], mphi = meta[2];
+  const ForceLocF floc = force_loc_f32(mphi);
   for (int q = threadIdx.x; q < w.nwin; q += kBigBlock) {
     const Q cp = coeff[mphi.offset + w.k_lo + q];
     const Q cr = coeff[mrho.offset + w.k_lo + q];
     Q x;
     x.x = cp.y;
-    x.y = cp.z;
-    x.z = cp.w;
+    x.y = kPre ? static_cast<R>(2) * cp.z : cp.z;
+    x.z = kPre ? static_cast<R>(3) * cp.w : cp.w;
     x.w = cr.y;
     V2 y;
-    y.x = cr.z;
-    y.y = cr.w;
+    y.x = kPre ? static_cast<R>(2) * cr.z : cr.z;
+    y.y = kPre ? static_cast<R>(3) * cr.w : cr.w;
     tA[q] = x;
     tB[q] = y;
     if (kEnergy) tE[q] = cp.x;
   }
+  auto deriv = [](R b, R c, R d, R t) -> R {
+    if constexpr (kPre)
+      return cubic_deriv_s(b, c, d, t);
+    else if constexpr (kF32)
+      return cubic_deriv_s(b, 2.0f * c, 3.0f * d, t);
+    else
+      return cubic_deriv<R>(b, c, d, t);
+  };
   const T* __restrict__ M2 = static_cast<const T*>(a.M2);
   T* hrec = reinterpret_cast<T*>(
       smem + window_bytes(w.nwin, sizeof(Q) + sizeof(V2) + (kEnergy ? sizeof(R) : 0)));
@@ -306,33 +321,56 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
       double dx, dy, dz;
       const double s2 = pair_delta<Mir, kIso>(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && s2 < rcut2) {
-        const PairR<R> pr = pair_r<R>(kIso ? s2 * u2 : s2);
-        R t;
-        const int idx = spline_locate<R>(mphi, pr.r, &t);
-        const int q = idx - w.k_lo;
-        // every lane reads the window (below it: its first interval); the
-        // rare pairs below the window then reload from the global table
+        const double r2 = kIso ? s2 * u2 : s2;
+        R t, rinv;
+        int idx, qs;
+        bool below;
+        // every lane reads the window (below it: its first interval); in
+        // the rare wave with a pair below the window every lane then
+        // re-reads its interval from the global table (the same values),
         // under a wave-uniform branch, which keeps the next group's loads
         // in flight when no lane needs it
-        const int qs = (!kWin || q >= 0) ? q : 0;
+        if constexpr (kF32) {
+          const PairT p = pair_t_f32(floc, r2, kWin ? static_cast<float>(w.k_lo) : 0.0f);
+          t = p.t;
+          rinv = p.rinv;
+          idx = p.idx;
+          below = kWin && p.below;
+          qs = idx - w.k_lo;  // idx >= k_lo
+        } else {
+          const PairR<R> pr = pair_r<R>(r2);
+          idx = spline_locate<R>(mphi, pr.r, &t);
+          rinv = pr.rinv;
+          const int q = idx - w.k_lo;
+          below = kWin && q < 0;
+          qs = below ? 0 : q;
+        }
         const Q x = tA[qs];
         const V2 y = tB[qs];
-        R dphi = cubic_deriv<R>(x.x, x.y, x.z, t);
-        R drho = cubic_deriv<R>(x.w, y.x, y.y, t);
+        R dphi = deriv(x.x, x.y, x.z, t);
+        R drho = deriv(x.w, y.x, y.y, t);
         R ephi = kEnergy ? cubic_value<R>(tE[qs], x.x, x.y, x.z, t) : static_cast<R>(0);
-        if (kWin && __any(q < 0)) {
-          if (q < 0) {
-            const Q cp = coeff[mphi.offset + idx];
-            const Q cr = coeff[mrho.offset + idx];
+        if (kWin && __builtin_amdgcn_ballot_w64(below) != 0) {
+          if constexpr (kF32) {  // the interval unclamped by the window
+            const PairT p = pair_t_f32(floc, r2);
+            t = p.t;
+            idx = p.idx;
+          }
+          const Q cp = coeff[mphi.offset + idx];  // c and d
+          const Q cr = coeff[mrho.offset + idx];
+          if constexpr (kF32) {
+            dphi = cubic_deriv_f(cp, t);
+            drho = cubic_deriv_f(cr, t);
+          } else {
             dphi = cubic_deriv<R>(cp.y, cp.z, cp.w, t);
             drho = cubic_deriv<R>(cr.y, cr.z, cr.w, t);
-            if (kEnergy) ephi = cubic_value<R>(cp.x, cp.y, cp.z, cp.w, t);
           }
+          if (kEnergy) ephi = cubic_value<R>(cp.x, cp.y, cp.z, cp.w, t);
         }
         if (kEnergy) pe += 0.5 * static_cast<double>(ephi);
         const R fpj = Mir::template w_real<R>(mj);
-        const double fpair = static_cast<double>(-(dphi + fpi * drho + fpj * drho) * pr.rinv);
-        if (kEnergy) vir += 0.5 * fpair * (kIso ? s2 * u2 : s2);
+        const double fpair = static_cast<double>(-(dphi + fpi * drho + fpj * drho) * rinv);
+        if (kEnergy) vir += 0.5 * fpair * r2;
         fx = fma(fpair, dx, fx);
         fy = fma(fpair, dy, fy);
         fz = fma(fpair, dz, fz);

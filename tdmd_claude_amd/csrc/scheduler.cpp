@@ -267,7 +267,9 @@ struct Engine final : EngineBase {
 
   // Interleaved derivative table of the several-element force pass (see
   // StepArgs::pairtab), from the tables' FP64 coefficients in the kernels'
-  // precision: the same values the per-spline quads hold.
+  // precision: the same values the per-spline quads hold; in mixed
+  // precision c and d are stored as 2c and 3d, scaled in float (the values
+  // the per-spline path forms at use).
   int build_pair_table() {
     const tdmd_eam_tables* t = S->tables;
     const int nt = t->ntypes;
@@ -278,8 +280,17 @@ struct Engine final : EngineBase {
       if (m[k].n != m[nt].n || m[k].dr != m[nt].dr || m[k].rmin != m[nt].rmin) return TDMD_OK;
     const int nint = m[nt].n - 1;
     std::vector<Q> h(static_cast<size_t>(nt) * nt * nint * 3);
+    constexpr bool kF32 = std::is_same_v<R, float>;
     auto co = [&](const std::vector<double>& v, int spline, int k) {
       return static_cast<R>(v[static_cast<size_t>(t->offsets[static_cast<size_t>(spline)] + k)]);
+    };
+    auto c2 = [&](int spline, int k) {
+      const R c = co(t->c, spline, k);
+      return kF32 ? static_cast<R>(2) * c : c;
+    };
+    auto d3 = [&](int spline, int k) {
+      const R d = co(t->d, spline, k);
+      return kF32 ? static_cast<R>(3) * d : d;
     };
     for (int ti = 0; ti < nt; ++ti)
       for (int tj = 0; tj < nt; ++tj) {
@@ -287,14 +298,14 @@ struct Engine final : EngineBase {
         for (int k = 0; k < nint; ++k) {
           Q* e = &h[(static_cast<size_t>(ti * nt + tj) * nint + k) * 3];
           e[0].x = co(t->b, sp, k);
-          e[0].y = co(t->c, sp, k);
-          e[0].z = co(t->d, sp, k);
+          e[0].y = c2(sp, k);
+          e[0].z = d3(sp, k);
           e[0].w = co(t->b, sj, k);
-          e[1].x = co(t->c, sj, k);
-          e[1].y = co(t->d, sj, k);
+          e[1].x = c2(sj, k);
+          e[1].y = d3(sj, k);
           e[1].z = co(t->b, si, k);
-          e[1].w = co(t->c, si, k);
-          e[2].x = co(t->d, si, k);
+          e[1].w = c2(si, k);
+          e[2].x = d3(si, k);
           e[2].y = e[2].z = e[2].w = static_cast<R>(0);
         }
       }
