@@ -302,8 +302,9 @@ typedef struct {
   int32_t mirror_fp64;    /* MIXED only: gather FP64 positions (1) or the */
                           /* 32-bit fixed-point mirror (0)                */
   int32_t max_neighbors;  /* 0 = sized from the first build               */
-  int32_t pair_unroll;    /* neighbour slots per pipelined group: 0 (=8),  */
-                          /* 4 or 8 (single-element EAM passes)           */
+  int32_t pair_unroll;    /* neighbour slots per pipelined group: 0 (=4   */
+                          /* mixed, 8 FP64), 4 or 8 (single-element EAM  */
+                          /* passes)                                      */
   int32_t lds_budget;     /* LDS bytes per workgroup for spline windows:  */
                           /* 0 = device limit                             */
   int32_t split_rebuild;  /* 1: neighbour rebuild as separate kernels;    */

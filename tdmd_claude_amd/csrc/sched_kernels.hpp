@@ -36,6 +36,14 @@ namespace sched {
 
 constexpr int kMaxTypes = 8;
 constexpr int kMaxSplines = 64;
+// Several-element EAM with at most this many elements: the density pass
+// stores F'(rho) in the force pass's records with its two lowest mantissa
+// bits replaced by the atom's element index (pack_typed), and the force pass
+// takes a neighbour's element from its record instead of gathering
+// types[j] (one scattered load per neighbour slot). F' is then used with a
+// relative perturbation below 4e-7 (FP32) / 7e-16 (FP64), the same value
+// for the atom's own term and its neighbours'.
+constexpr int kPackedTypes = 4;
 
 struct MirrorParams {
   double lo[3];
@@ -86,6 +94,14 @@ struct MirrorF64 {
   template <class R>
   __device__ __forceinline__ static double pack_real(R v) {
     return static_cast<double>(v);
+  }
+  // v with the element index t (0..3) in its two lowest mantissa bits
+  template <class R>
+  __device__ __forceinline__ static double pack_typed(R v, int t) {
+    return __longlong_as_double((__double_as_longlong(static_cast<double>(v)) & ~3ll) | t);
+  }
+  __device__ __forceinline__ static int packed_type(const T& m) {
+    return static_cast<int>(__double_as_longlong(m.w) & 3);
   }
 };
 
@@ -145,6 +161,12 @@ struct MirrorFixed {
   __device__ __forceinline__ static int pack_real(R v) {
     return __float_as_int(static_cast<float>(v));
   }
+  // v with the element index t (0..3) in its two lowest mantissa bits
+  template <class R>
+  __device__ __forceinline__ static int pack_typed(R v, int t) {
+    return (__float_as_int(static_cast<float>(v)) & ~3) | t;
+  }
+  __device__ __forceinline__ static int packed_type(const T& m) { return m.w & 3; }
 };
 
 // A batch of independent boxes (replicas) advanced together: the same

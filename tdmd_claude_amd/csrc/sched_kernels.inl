@@ -458,8 +458,13 @@ k_eam_density(StepArgs a, const SplineMeta* __restrict__ meta,
   }
   if constexpr (kLanes > 1) rho = lane_sum<kLanes>(rho);  // every lane takes part
   if (i < a.natoms && part == 0) {
-    const SplineVal<R> F = spline_both<R>(sm[Mir::type_of(mi)], coeff, rho);
-    static_cast<T*>(a.M2)[i] = Mir::with_w(mi, Mir::template pack_real<R>(F.df));
+    const int ti = Mir::type_of(mi);
+    const SplineVal<R> F = spline_both<R>(sm[ti], coeff, rho);
+    // two to kPackedTypes elements: the force pass reads a neighbour's
+    // element from F'(rho)'s two lowest mantissa bits
+    const bool typed = a.ntypes > 1 && a.ntypes <= kPackedTypes;
+    static_cast<T*>(a.M2)[i] = Mir::with_w(mi, typed ? Mir::template pack_typed<R>(F.df, ti)
+                                                     : Mir::template pack_real<R>(F.df));
     e = static_cast<double>(F.f);
   }
   if (kEnergy) emit_partial(e, a.partials, blk);
@@ -496,7 +501,8 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
       double dx, dy, dz;
       const double r2 = Mir::delta(mi, mj, mpi, &dx, &dy, &dz);
       if (in_row && r2 < rc_sq) {
-        const int tj = kMulti ? a.types[j] - 1 : 0;
+        const int tj = kMulti ? (a.ntypes <= kPackedTypes ? Mir::packed_type(mj) : a.types[j] - 1)
+                              : 0;
         const int pidx = kMulti ? phi_index(ti, tj, nt) : 0;
         R dphi, drho_j, drho_i, rinv;
         if constexpr (kF32) {
