@@ -973,33 +973,46 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
         // test where the prefilter is too close to call
         const int4* __restrict__ M = static_cast<const int4*>(a.M);
         for (int j0 = start; j0 < end; j0 += 4) {
-          int4 mq[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) mq[q] = M[min(j0 + q, end - 1)];
+          // The group's four prefilter decisions in straight-line code, so
+          // the four record loads are issued together and consumed there
+          // (with the distance computed under each candidate's branch the
+          // compiler re-issued the first load inside its branch and waited
+          // on it); the exact test of the rare in-band candidates and the
+          // appends follow.
+          bool in[4], near[4], band[4];
+          bool any_band = false;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int j = j0 + q;
-            if (j < end && j != i) {
-              const float r2f = pre_r2(mq[q], mi_rec, pb);
-              bool in, near;
-              if ((r2f >= pb.lo_list && r2f <= pb.hi_list) ||
-                  (r2f >= pb.lo_near && r2f <= pb.hi_near)) {
-                const double r2 = exact_r2(j);
-                in = r2 < r_list_sq;
-                near = r2 < near_sq;
-              } else {
-                in = r2f < pb.lo_list;
-                near = r2f < pb.lo_near;
+            const float r2f = pre_r2(M[min(j, end - 1)], mi_rec, pb);
+            // bitwise, not short-circuit, so no branch splits the group
+            const bool valid = (j < end) & (j != i);
+            band[q] = valid & (((r2f >= pb.lo_list) & (r2f <= pb.hi_list)) |
+                               ((r2f >= pb.lo_near) & (r2f <= pb.hi_near)));
+            in[q] = valid & (r2f < pb.lo_list);
+            near[q] = valid & (r2f < pb.lo_near);
+            any_band |= band[q];
+          }
+          if (any_band) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (band[q]) {
+                const double r2 = exact_r2(j0 + q);
+                in[q] = r2 < r_list_sq;
+                near[q] = r2 < near_sq;
               }
-              if (in) {
-                if (!kCount && nn + nf < cap) {
-                  if (near)
-                    nbr[ell(i, nn++, cap)] = j;
-                  else
-                    nbr[ell(i, cap - 1 - nf++, cap)] = j;
-                }
-                ++count;
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (in[q]) {
+              if (!kCount && nn + nf < cap) {
+                if (near[q])
+                  nbr[ell(i, nn++, cap)] = j0 + q;
+                else
+                  nbr[ell(i, cap - 1 - nf++, cap)] = j0 + q;
               }
+              ++count;
             }
           }
         }
@@ -1562,10 +1575,13 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, int32_t* flags) {
 
 constexpr int kRebuildBlock = 1024;  // workgroup of the fused rebuild
 
-// Registers capped for 8 waves per SIMD: two workgroups per CU (the halo
-// phase's 64 KB of LDS allows two).
+// Registers are not capped: the list search's candidate groups need more
+// than the 64 VGPRs that eight waves per SIMD (two workgroups per CU) would
+// leave, and spilled under that cap (1M-atom rebuild 2.14 ms capped, 2.10
+// uncapped; 0.67 / 0.58 ms at 256,000 atoms). The grid is as many
+// workgroups as the device holds at once (one or two per CU).
 template <class R, class Mir>
-__global__ void __launch_bounds__(kRebuildBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(kRebuildBlock)
 k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double near_sq,
                 int32_t* nbr, int32_t* cnt, GridBarrier* bar, int64_t* rebuilds, int forced) {
   // uniform: nothing writes the flag before the last phase
