@@ -1312,119 +1312,169 @@ __global__ void __launch_bounds__(kBlock) k_rebuild_done(int32_t* flags, int32_t
 //
 // Built after every rebuild by one workgroup per block at a time -- as the
 // last phase of the fused rebuild, or by k_halo after a rebuild by separate
-// kernels (flags[4]): an LDS hash set collects the distinct indices of the block's rows,
-// a bitonic sort orders them, the list is written out, and every row slot
-// is rewritten as its position in the list (binary search). A halo larger
-// than stage_cap (or than kHaloMax) leaves the block on its 32-bit rows.
-// k_halo's last workgroup to finish clears flags[4].
+// kernels (flags[4]). The set is kept as 64-atom chunks: an LDS hash maps a
+// chunk id (j >> 6) to a 64-bit mask of its atoms present. Atoms are in
+// cell order, so a row names its neighbours in runs of a few chunks (a
+// stencil cell's atoms are adjacent indices): a thread ORs one mask per run
+// into the set instead of inserting every slot. The chunk ids are sorted
+// (a few hundred at most), a scan of their popcounts gives each chunk's
+// first list position, the list goes out, and a row slot j becomes
+// base(j >> 6) + popcount(mask(j >> 6) below bit j & 63). A halo larger
+// than stage_cap (or than kHaloMax), or spread over more than kChunkMax
+// chunks, leaves the block on its 32-bit rows. k_halo's last workgroup to
+// finish clears flags[4].
 // ---------------------------------------------------------------------------
-constexpr int kHaloHashBits = 13;
-constexpr int kHaloHash = 1 << kHaloHashBits;  // hash slots
-constexpr int kHaloMax = 6144;                 // distinct atoms a halo may hold
-static_assert(kHaloMax <= 65536 && 4 * kHaloMax <= 3 * kHaloHash, "halo sizing");
-
-// Multiplicative (Fibonacci) hash of an atom index onto the set's slots.
-__device__ __forceinline__ unsigned halo_hash(int j) {
-  return (static_cast<unsigned>(j) * 2654435761u) >> (32 - kHaloHashBits);
-}
-
-// Insert j into the open-addressing set `key` (empty = -1), linear
-// probing. Counts new keys in *nd and raises
-// *ovf once more than kHaloMax are held; inserts stop then, so the table
-// never holds more than kHaloMax + (threads of the workgroup) keys and a
-// probe always meets its key or an empty slot.
-__device__ __forceinline__ void halo_insert(int32_t* key, int j, int* nd, volatile int* ovf) {
-  unsigned h = halo_hash(j);
-  for (int p = 0; p < kHaloHash; ++p) {
-    const int old = atomicCAS(&key[h], -1, j);
-    if (old == j) return;
-    if (old == -1) {
-      if (atomicAdd(nd, 1) >= kHaloMax) *ovf = 1;
-      return;
-    }
-    h = (h + 1) & (kHaloHash - 1);
-  }
-  *ovf = 1;
-}
+constexpr int kHaloMax = 6144;                 // distinct atoms a staged halo may hold
+constexpr int kChunkBits = 10;
+constexpr int kChunkSlots = 1 << kChunkBits;   // hash slots of the chunk set
+constexpr int kChunkMax = 768;                 // distinct chunks a halo may touch
+static_assert(kHaloMax <= 65536 && kChunkMax <= kChunkSlots, "halo sizing");
 
 constexpr int kHaloBlock = 1024;  // atoms per staged block (= the passes' workgroup)
+static_assert(kChunkSlots == kHaloBlock, "one chunk slot per thread in the compaction");
 
-struct HaloLds {
-  int32_t key[kHaloHash];
-  int32_t srt[kHaloHash];
-  int nd, fill, ovf;
+// One slot of the chunk set: chunk id (empty = -1), first list position of
+// its atoms, mask of its atoms present. 16 bytes: a probe is one LDS read.
+struct alignas(16) HaloChunk {
+  int32_t key;
+  int32_t base;
+  uint64_t mask;
 };
 
-// Hash slot of j, which the set holds.
-__device__ __forceinline__ int halo_find(const int32_t* key, int j) {
-  unsigned h = halo_hash(j);
-  while (key[h] != j) h = (h + 1) & (kHaloHash - 1);
-  return static_cast<int>(h);
+struct HaloLds {
+  HaloChunk ent[kChunkSlots];
+  int32_t srt[kChunkSlots];   // chunk ids, sorted
+  int32_t slot[kChunkSlots];  // hash slot of srt[p]
+  int32_t wsum[kHaloBlock / 64];
+  int nc, fill, ovf;
+};
+
+// Multiplicative (Fibonacci) hash of a chunk id onto the set's slots.
+__device__ __forceinline__ unsigned chunk_hash(int c) {
+  return (static_cast<unsigned>(c) * 2654435761u) >> (32 - kChunkBits);
+}
+
+// OR mask m into chunk c of the set (linear probing; a slot's key goes
+// from -1 to its chunk once, by CAS). Counts new chunks in *nc; returns
+// false once more than kChunkMax are held (the caller stops inserting) or
+// the probe finds no slot.
+__device__ __forceinline__ bool chunk_or(HaloChunk* ent, int c, uint64_t m, int* nc) {
+  unsigned h = chunk_hash(c);
+  for (int p = 0; p < kChunkSlots; ++p) {
+    int k = __hip_atomic_load(&ent[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == -1) {
+      k = atomicCAS(&ent[h].key, -1, c);
+      if (k == -1) {
+        if (atomicAdd(nc, 1) >= kChunkMax) return false;
+        k = c;
+      }
+    }
+    if (k == c) {
+      atomicOr(reinterpret_cast<unsigned long long*>(&ent[h].mask),
+               static_cast<unsigned long long>(m));
+      return true;
+    }
+    h = (h + 1) & (kChunkSlots - 1);
+  }
+  return false;
+}
+
+// Slot of chunk c, which the set holds, read whole: {key, base, mask lo,
+// mask hi}. The probe is bounded by the table size all the same.
+__device__ __forceinline__ int4 chunk_find(const HaloChunk* ent, int c) {
+  unsigned h = chunk_hash(c);
+  int4 e = *reinterpret_cast<const int4*>(&ent[h]);
+  for (int p = 1; e.x != c && p < kChunkSlots; ++p) {
+    h = (h + 1) & (kChunkSlots - 1);
+    e = *reinterpret_cast<const int4*>(&ent[h]);
+  }
+  return e;
+}
+
+// List position of atom j, whose chunk the set holds.
+__device__ __forceinline__ uint32_t halo_pos(const HaloChunk* ent, int j) {
+  const int4 e = chunk_find(ent, j >> 6);
+  const uint64_t m = static_cast<uint32_t>(e.z) | (static_cast<uint64_t>(static_cast<uint32_t>(e.w)) << 32);
+  const uint64_t below = (uint64_t{1} << (j & 63)) - 1u;
+  return static_cast<uint32_t>(e.y + __popcll(m & below));
 }
 
 // Halos of blocks blockIdx.x, blockIdx.x + gridDim.x, ... (< nblocks), by a
 // workgroup of kHaloBlock threads; its LDS is `L`.
 __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, HaloLds& L) {
   constexpr int nt = kHaloBlock;
-  constexpr int kPer = (kHaloMax + nt - 1) / nt;  // sorted keys per thread
   const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
   const int cap = a.cap;
   int32_t* hj = const_cast<int32_t*>(a.hj);
   int32_t* hn = const_cast<int32_t*>(a.hn);
   int32_t* hmode = const_cast<int32_t*>(a.hmode);
   uint16_t* nbr16 = const_cast<uint16_t*>(a.nbr16);
+  const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
   for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    for (int t = tid; t < kHaloHash; t += nt) L.key[t] = -1;
+    L.ent[tid].key = -1;
+    L.ent[tid].mask = 0;
     if (tid == 0) {
-      L.nd = 0;
+      L.nc = 0;
       L.fill = 0;
       L.ovf = 0;
     }
     __syncthreads();
-    volatile int* vovf = &L.ovf;
-    for (int t = tid; t < kHaloBlock && !*vovf; t += nt) {
-      const int i = b * kHaloBlock + t;
-      if (i >= a.natoms) break;
-      halo_insert(L.key, i, &L.nd, vovf);  // row padding names i itself
-      // the row in quads (four adjacent slots per load), the next quad
-      // loaded while this one is inserted; slots past the count in the last
-      // quad are padding (i, already present)
+    const int i = b * kHaloBlock + tid;
+    if (i < a.natoms) {
+      // runs of one chunk are merged in (cc, m) and ORed in when the chunk
+      // changes; the row in quads, the next quad loaded while this one is
+      // merged; slots past the count in the last quad are padding (i, whose
+      // chunk starts the first run, since row padding names i itself)
+      int cc = i >> 6;
+      uint64_t m = uint64_t{1} << (i & 63);
+      bool ok = true;
       const int nq = (a.cnt[i] + 3) >> 2;
-      const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
       int4 cur = nq > 0 ? q4[ell(i, 0, cap) >> 2] : make_int4(i, i, i, i);
-      for (int q = 0; q < nq && !*vovf; ++q) {
+      for (int q = 0; q < nq && ok; ++q) {
         const int4 nxt = q + 1 < nq ? q4[ell(i, 4 * (q + 1), cap) >> 2] : cur;
-        halo_insert(L.key, cur.x, &L.nd, vovf);
-        if (!*vovf) halo_insert(L.key, cur.y, &L.nd, vovf);
-        if (!*vovf) halo_insert(L.key, cur.z, &L.nd, vovf);
-        if (!*vovf) halo_insert(L.key, cur.w, &L.nd, vovf);
+        auto merge = [&](int j) {
+          if ((j >> 6) != cc) {
+            ok = ok && chunk_or(L.ent, cc, m, &L.nc);
+            cc = j >> 6;
+            m = 0;
+          }
+          m |= uint64_t{1} << (j & 63);
+        };
+        merge(cur.x);
+        merge(cur.y);
+        merge(cur.z);
+        merge(cur.w);
         cur = nxt;
       }
+      if (ok) ok = chunk_or(L.ent, cc, m, &L.nc);
+      if (!ok) L.ovf = 1;
     }
     __syncthreads();
-    const int H = L.nd;
-    if (L.ovf || H > a.stage_cap) {  // the same in every thread
+    if (L.ovf) {  // the same in every thread
       if (tid == 0) {
-        hn[b] = H;
+        hn[b] = kHaloMax + 1;
         hmode[b] = 1;
       }
       __syncthreads();  // LDS is reused by the next block
       continue;
     }
-    for (int t = tid; t < kHaloHash; t += nt) {
-      const int v = L.key[t];
-      if (v != -1) L.srt[atomicAdd(&L.fill, 1)] = v;
+    // compaction: one slot per thread
+    {
+      const int k = L.ent[tid].key;
+      if (k != -1) L.srt[atomicAdd(&L.fill, 1)] = k;
     }
-    int P = 1;
-    while (P < H) P <<= 1;
     __syncthreads();
-    for (int t = H + tid; t < P; t += nt) L.srt[t] = 0x7fffffff;
+    const int C = L.fill;
+    int P = 1;
+    while (P < C) P <<= 1;
+    if (tid >= C && tid < P) L.srt[tid] = 0x7fffffff;
     __syncthreads();
     // bitonic sort of srt[0, P), ascending
     for (int size = 2; size <= P; size <<= 1) {
       for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int t = tid; t < (P >> 1); t += nt) {
-          const int lo = 2 * t - (t & (stride - 1));
+        if (tid < (P >> 1)) {
+          const int lo = 2 * tid - (tid & (stride - 1));
           const int hi = lo + stride;
           const bool up = (lo & size) == 0;
           const int x = L.srt[lo], y = L.srt[hi];
@@ -1436,39 +1486,62 @@ __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, Halo
         __syncthreads();
       }
     }
-    // The list goes out; then srt is reused to hold each key's list
-    // position at the key's hash slot (every thread first takes its keys
-    // into registers), so a row slot's position is one hash probe away.
-    int v[kPer];
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-      const int l = q * nt + tid;
-      v[q] = l < H ? L.srt[l] : -1;
-      if (l < H) hj[static_cast<size_t>(b) * kHaloMax + l] = v[q];
+    // exclusive scan of the sorted chunks' popcounts: each chunk's base
+    int h = 0, n = 0;
+    if (tid < C) {
+      h = static_cast<int>(chunk_hash(L.srt[tid]));
+      while (L.ent[h].key != L.srt[tid]) h = (h + 1) & (kChunkSlots - 1);
+      n = __popcll(L.ent[h].mask);
+    }
+    int incl = n;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) L.wsum[w] = incl;
+    __syncthreads();
+    int base = incl - n, H = 0;
+    for (int k = 0; k < nt / 64; ++k) {
+      const int s = L.wsum[k];
+      base += k < w ? s : 0;
+      H += s;
+    }
+    if (H > a.stage_cap || H > kHaloMax) {  // the same in every thread
+      if (tid == 0) {
+        hn[b] = H;
+        hmode[b] = 1;
+      }
+      __syncthreads();
+      continue;
+    }
+    if (tid < C) {
+      L.ent[h].base = base;
+      L.slot[tid] = h;
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kPer; ++q)
-      if (v[q] >= 0) L.srt[halo_find(L.key, v[q])] = q * nt + tid;
-    __syncthreads();
-    {
-      const int i = b * kHaloBlock + tid;
-      if (i < a.natoms) {
-        // the padded row in quads; a quad's four 16-bit positions are
-        // adjacent in the 16-bit layout (eight slots per 16 bytes)
-        const int mq = ((a.cnt[i] + 7) & ~7) >> 2;
-        const int4* __restrict__ q4 = reinterpret_cast<const int4*>(a.nbr);
-        int4 cur = mq > 0 ? q4[ell(i, 0, cap) >> 2] : make_int4(i, i, i, i);
-        for (int q = 0; q < mq; ++q) {
-          const int4 nxt = q + 1 < mq ? q4[ell(i, 4 * (q + 1), cap) >> 2] : cur;
-          const uint32_t p0 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.x)]);
-          const uint32_t p1 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.y)]);
-          const uint32_t p2 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.z)]);
-          const uint32_t p3 = static_cast<uint16_t>(L.srt[halo_find(L.key, cur.w)]);
-          *reinterpret_cast<uint2*>(nbr16 + ell16(i, 4 * q, cap)) =
-              make_uint2(p0 | (p1 << 16), p2 | (p3 << 16));
-          cur = nxt;
-        }
+    // the list: one wavefront per chunk, lane l writes atom 64 c + l if present
+    for (int p = w; p < C; p += nt / 64) {
+      const HaloChunk e = L.ent[L.slot[p]];
+      if ((e.mask >> lane) & 1u) {
+        const uint64_t below = (uint64_t{1} << lane) - 1u;
+        hj[static_cast<size_t>(b) * kHaloMax + e.base + __popcll(e.mask & below)] =
+            e.key * 64 + lane;
+      }
+    }
+    if (i < a.natoms) {
+      // the padded row in quads; a quad's four 16-bit positions are
+      // adjacent in the 16-bit layout (eight slots per 16 bytes)
+      const int mq = ((a.cnt[i] + 7) & ~7) >> 2;
+      int4 cur = mq > 0 ? q4[ell(i, 0, cap) >> 2] : make_int4(i, i, i, i);
+      for (int q = 0; q < mq; ++q) {
+        const int4 nxt = q + 1 < mq ? q4[ell(i, 4 * (q + 1), cap) >> 2] : cur;
+        const uint32_t p0 = halo_pos(L.ent, cur.x);
+        const uint32_t p1 = halo_pos(L.ent, cur.y);
+        const uint32_t p2 = halo_pos(L.ent, cur.z);
+        const uint32_t p3 = halo_pos(L.ent, cur.w);
+        *reinterpret_cast<uint2*>(nbr16 + ell16(i, 4 * q, cap)) =
+            make_uint2(p0 | (p1 << 16), p2 | (p3 << 16));
+        cur = nxt;
       }
     }
     if (tid == 0) {
