@@ -45,6 +45,11 @@ constexpr int kMaxSplines = 64;
 // for the atom's own term and its neighbours'.
 constexpr int kPackedTypes = 4;
 
+// Spare records allocated past the last atom in each gather-record buffer:
+// the list search loads a cell's candidates in whole groups of eight from
+// the cell's first atom, masking the slots past the cell's end.
+constexpr int kRecPad = 8;
+
 struct MirrorParams {
   double lo[3];
   double q[3];     // fixed-point units per A (MirrorFixed)

@@ -675,7 +675,10 @@ struct GridArgs {
   int32_t* sidx;           // [N] new slot -> old index
   int32_t* orig;           // [N] upload index
   int32_t* done;           // completion counter
+  int32_t* wsum;           // [kMaxScanGroups] per-workgroup cell sums (fused rebuild)
 };
+
+constexpr int kMaxScanGroups = 4096;  // workgroups of the fused rebuild, at most
 
 
 struct PermArgs {
@@ -740,6 +743,28 @@ __device__ __forceinline__ void cell_coords(const CellGrid& cg, double x, double
 // ---- per-item bodies, shared by the separate kernels (first build) and
 // the fused rebuild kernel ----
 
+// Adds one to ctr[key] for each active lane of the wavefront with one atomic
+// per distinct key (atoms are in cell order from the previous sort, so a
+// wave's lanes share a few cells and per-lane atomics on one address
+// serialise); returns the counter's value before this lane's increment, in
+// lane order within each key. Every active lane of the wave must call it.
+__device__ __forceinline__ int wave_add_one(int32_t* ctr, int key) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  unsigned long long todo = __ballot(1);
+  int old = 0;
+  while (todo) {
+    const int leader = __ffsll(todo) - 1;
+    const int k = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(key == k) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ctr[k], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if ((same >> lane) & 1ull) old = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return old;
+}
+
 template <class Mir>
 __device__ __forceinline__ void bin_atom(const StepArgs& a, const GridArgs& g, int i) {
   int c[3];
@@ -747,7 +772,7 @@ __device__ __forceinline__ void bin_atom(const StepArgs& a, const GridArgs& g, i
   cell_coords(cg, a.px[i], a.py[i], a.pz[i], c);
   const int rank = g.rank_of[cg.off + (c[2] * cg.n[1] + c[1]) * cg.n[0] + c[0]];
   g.key[i] = rank;
-  atomicAdd(&g.ccount[rank], 1);
+  (void)wave_add_one(g.ccount, rank);
 }
 
 // Exclusive scan of the cell counts by one workgroup of BLOCK threads
@@ -777,9 +802,67 @@ __device__ __forceinline__ void scan_cells(const GridArgs& g) {
   }
 }
 
+// (The order of a cell's atoms after the scatter is the atomics' order;
+// the ordering pass then sorts each cell by upload index.)
 __device__ __forceinline__ void scatter_atom(const GridArgs& g, int i) {
   const int r = g.key[i];
-  g.sidx[g.cstart[r] + atomicAdd(&g.cursor[r], 1)] = i;
+  g.sidx[g.cstart[r] + wave_add_one(g.cursor, r)] = i;
+}
+
+// Inclusive scan of v over the workgroup (BLOCK threads); *total gets the
+// workgroup's sum. Uses `ws` (BLOCK / 64 ints of LDS); ends synchronised.
+template <int BLOCK>
+__device__ __forceinline__ int block_scan_incl(int v, int* total, int* ws) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  if (lane == 63) ws[w] = incl;
+  __syncthreads();
+  int before = 0, sum = 0;
+  for (int k = 0; k < BLOCK / 64; ++k) {
+    const int x = ws[k];
+    before += k < w ? x : 0;
+    sum += x;
+  }
+  __syncthreads();  // ws is reused by the next call
+  *total = sum;
+  return before + incl;
+}
+
+// The exclusive cell scan by the whole grid of the fused rebuild, in two
+// phases with a grid barrier between them: workgroup w owns a contiguous
+// chunk of cells; phase 1 sums its chunk into wsum[w]; phase 2 adds the sums
+// of the chunks before it and scans its chunk. The same starts as
+// scan_cells, which one workgroup computes alone.
+template <int BLOCK>
+__device__ __forceinline__ void scan_cells_grid(const GridArgs& g, int phase) {
+  __shared__ int ws[BLOCK / 64];
+  const int W = gridDim.x;
+  const int chunk = (g.ncells + W - 1) / W;
+  const int c0 = min(static_cast<int>(blockIdx.x) * chunk, g.ncells);
+  const int c1 = min(c0 + chunk, g.ncells);
+  int total = 0;
+  if (phase == 1) {
+    int v = 0;
+    for (int c = c0 + threadIdx.x; c < c1; c += BLOCK) v += g.ccount[c];
+    (void)block_scan_incl<BLOCK>(v, &total, ws);
+    if (threadIdx.x == 0) g.wsum[blockIdx.x] = total;
+    return;
+  }
+  int v = 0;
+  for (int k = threadIdx.x; k < static_cast<int>(blockIdx.x); k += BLOCK) v += g.wsum[k];
+  (void)block_scan_incl<BLOCK>(v, &total, ws);
+  int run = total;  // cells before this chunk
+  for (int c = c0; c < c1; c += BLOCK) {
+    const int cc = c + threadIdx.x;
+    const int n = cc < c1 ? g.ccount[cc] : 0;
+    const int incl = block_scan_incl<BLOCK>(n, &total, ws);
+    if (cc < c1) g.cstart[cc] = run + incl - n;
+    run += total;
+  }
 }
 
 // Deterministic order inside a cell: by upload index.
@@ -873,28 +956,40 @@ __device__ __forceinline__ void copyback_atom(const StepArgs& a, const GridArgs&
 // candidates per million atoms) the exact test decides.
 constexpr double kPreMargin = 1e-5;
 
+// When the three axes share one unit u (cubic boxes), r^2 is formed in
+// units^2 and the bounds are divided by u^2 instead: three products fewer
+// per candidate, the same relative error.
 struct PreBounds {
   float lo_list, hi_list, lo_near, hi_near;
   float ux, uy, uz;
+  bool iso;
 };
 
 __device__ __forceinline__ PreBounds pre_bounds(const MirrorParams& mp, double r_list_sq,
                                                 double near_sq) {
   PreBounds b;
-  b.lo_list = static_cast<float>(r_list_sq * (1.0 - kPreMargin));
-  b.hi_list = static_cast<float>(r_list_sq * (1.0 + kPreMargin));
-  b.lo_near = static_cast<float>(near_sq * (1.0 - kPreMargin));
-  b.hi_near = static_cast<float>(near_sq * (1.0 + kPreMargin));
+  b.iso = mp.unit[0] == mp.unit[1] && mp.unit[1] == mp.unit[2];
+  const double s = b.iso ? 1.0 / (mp.unit[0] * mp.unit[0]) : 1.0;
+  b.lo_list = static_cast<float>(r_list_sq * s * (1.0 - kPreMargin));
+  b.hi_list = static_cast<float>(r_list_sq * s * (1.0 + kPreMargin));
+  b.lo_near = static_cast<float>(near_sq * s * (1.0 - kPreMargin));
+  b.hi_near = static_cast<float>(near_sq * s * (1.0 + kPreMargin));
   b.ux = static_cast<float>(mp.unit[0]);
   b.uy = static_cast<float>(mp.unit[1]);
   b.uz = static_cast<float>(mp.unit[2]);
   return b;
 }
 
+template <bool kIsoU>
 __device__ __forceinline__ float pre_r2(const int4& mj, const int4& mi, const PreBounds& b) {
-  const float dx = static_cast<float>(MirrorFixed::wdiff(mj.x, mi.x)) * b.ux;
-  const float dy = static_cast<float>(MirrorFixed::wdiff(mj.y, mi.y)) * b.uy;
-  const float dz = static_cast<float>(MirrorFixed::wdiff(mj.z, mi.z)) * b.uz;
+  float dx = static_cast<float>(MirrorFixed::wdiff(mj.x, mi.x));
+  float dy = static_cast<float>(MirrorFixed::wdiff(mj.y, mi.y));
+  float dz = static_cast<float>(MirrorFixed::wdiff(mj.z, mi.z));
+  if constexpr (!kIsoU) {
+    dx *= b.ux;
+    dy *= b.uy;
+    dz *= b.uz;
+  }
   return dx * dx + dy * dy + dz * dz;
 }
 
@@ -936,8 +1031,13 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
   }
   // One copy of the search per image mode, so the loop body has no branch
   // between a group's coordinate loads and they issue together.
-  auto search = [&](auto images) {
+  // Row slots are addressed from the row's first slot: slot k is
+  // (k / 4) * 256 + k % 4 elements on (the ELL layout of ell()).
+  int32_t* const rowp = kCount ? nullptr : nbr + ell(i, 0, cap);
+  auto slot = [&](int k) -> int32_t& { return rowp[(k >> 2) * 256 + (k & 3)]; };
+  auto search = [&](auto images, auto iso_units) {
     constexpr bool kImg = decltype(images)::value;
+    constexpr bool kIsoU = decltype(iso_units)::value;
     for (int t = 0; t < 27; ++t) {
       const int r = around[t];
       const int start = g.cstart[r], end = start + g.ccount[r];
@@ -971,7 +1071,7 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
       if constexpr (Mir::kFixed) {
         // one 16-byte record per candidate, prefiltered in FP32; the exact
         // test where the prefilter is too close to call
-        const int4* __restrict__ M = static_cast<const int4*>(a.M);
+        const char* __restrict__ Mb = static_cast<const char*>(a.M);
         for (int j0 = start; j0 < end; j0 += 8) {
           // A group of eight candidates: their prefilter decisions in
           // straight-line code, so the eight record loads are issued together
@@ -979,19 +1079,26 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
           // candidate's branch the compiler re-issued the first load inside
           // its branch and waited on it); the exact test of the rare in-band
           // candidates and the appends follow. (Groups of four: 1M-atom
-          // rebuild 2.09 ms; eight 2.00; sixteen 2.06.)
+          // rebuild 2.09 ms; eight 2.00; sixteen 2.06.) The eight records
+          // are read from the group's first at fixed offsets, also past the
+          // cell's end (the buffer has kRecPad spare records at its end);
+          // those slots are masked.
+          const int4* __restrict__ g8 =
+              reinterpret_cast<const int4*>(Mb + (static_cast<uint32_t>(j0) << 4));
           bool in[8], near[8], band[8];
           bool any_band = false;
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
             const int j = j0 + q;
-            const float r2f = pre_r2(M[min(j, end - 1)], mi_rec, pb);
-            // bitwise, not short-circuit, so no branch splits the group
+            const float r2f = pre_r2<kIsoU>(g8[q], mi_rec, pb);
+            // bitwise, not short-circuit, so no branch splits the group;
+            // a band is [lo, hi] around a bound: below hi and not below lo
             const bool valid = (j < end) & (j != i);
-            band[q] = valid & (((r2f >= pb.lo_list) & (r2f <= pb.hi_list)) |
-                               ((r2f >= pb.lo_near) & (r2f <= pb.hi_near)));
-            in[q] = valid & (r2f < pb.lo_list);
-            near[q] = valid & (r2f < pb.lo_near);
+            const bool list_lo = r2f < pb.lo_list, list_hi = r2f <= pb.hi_list;
+            const bool near_lo = r2f < pb.lo_near, near_hi = r2f <= pb.hi_near;
+            band[q] = valid & ((list_hi & !list_lo) | (near_hi & !near_lo));
+            in[q] = valid & list_lo;
+            near[q] = valid & near_lo;
             any_band |= band[q];
           }
           if (any_band) {
@@ -1009,9 +1116,9 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
             if (in[q]) {
               if (!kCount && nn + nf < cap) {
                 if (near[q])
-                  nbr[ell(i, nn++, cap)] = j0 + q;
+                  slot(nn++) = j0 + q;
                 else
-                  nbr[ell(i, cap - 1 - nf++, cap)] = j0 + q;
+                  slot(cap - 1 - nf++) = j0 + q;
               }
               ++count;
             }
@@ -1051,9 +1158,9 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
           if (j < end && j != i && r2q[q] < r_list_sq) {
             if (!kCount && nn + nf < cap) {
               if (r2q[q] < near_sq)
-                nbr[ell(i, nn++, cap)] = j;
+                slot(nn++) = j;
               else
-                nbr[ell(i, cap - 1 - nf++, cap)] = j;
+                slot(cap - 1 - nf++) = j;
             }
             ++count;
           }
@@ -1066,22 +1173,30 @@ __device__ __forceinline__ int nlist_atom(const StepArgs& a, const GridArgs& g, 
     images = a.boxes[a.rep[i]].images != 0;
   else
     images = g.images != 0;
-  if (images)
-    search(std::true_type{});
-  else
-    search(std::false_type{});
+  // (pb.iso is false unless the records are fixed-point)
+  if (images) {
+    if (pb.iso)
+      search(std::true_type{}, std::true_type{});
+    else
+      search(std::true_type{}, std::false_type{});
+  } else {
+    if (pb.iso)
+      search(std::false_type{}, std::true_type{});
+    else
+      search(std::false_type{}, std::false_type{});
+  }
   if (!kCount) {
     // Shell entries sit at [cap - nf, cap); the gap [nn, cap - nf) has
     // room for the m that must move (the rest already lie in [nn, nn+nf)).
     const int gap = cap - nn - nf;
     const int m = nf < gap ? nf : gap;
-    for (int q = 0; q < m; ++q) nbr[ell(i, nn + q, cap)] = nbr[ell(i, cap - 1 - q, cap)];
+    for (int q = 0; q < m; ++q) slot(nn + q) = slot(cap - 1 - q);
     const int n = nn + nf;  // == min(count, cap)
     cnt[i] = n;
     a.cntn[i] = nn;
     // Pad the row to a multiple of 8 with the atom itself, so unrolled
     // readers can load whole groups of slots (cap is a multiple of 8).
-    for (int k = n; k < ((n + 7) & ~7); ++k) nbr[ell(i, k, cap)] = i;
+    for (int k = n; k < ((n + 7) & ~7); ++k) slot(k) = i;
   }
   return count;
 }
@@ -1162,7 +1277,8 @@ __device__ __forceinline__ int nlist_atom_group(const StepArgs& a, const GridArg
       bool in_list, in_near;
       if constexpr (Mir::kFixed) {
         // the record prefilter of nlist_atom, the exact test in its band
-        const float r2f = pre_r2(static_cast<const int4*>(a.M)[jj], mi_rec, pb);
+        const float r2f = pb.iso ? pre_r2<true>(static_cast<const int4*>(a.M)[jj], mi_rec, pb)
+                                 : pre_r2<false>(static_cast<const int4*>(a.M)[jj], mi_rec, pb);
         if ((r2f >= pb.lo_list && r2f <= pb.hi_list) ||
             (r2f >= pb.lo_near && r2f <= pb.hi_near)) {
           const double r2 = exact_r2();
@@ -1670,7 +1786,9 @@ k_rebuild_fused(StepArgs a, GridArgs g, PermArgs t, double r_list_sq, double nea
   if (!grid_barrier(bar, a.flags)) return;
   for (int i = tid; i < N; i += nthr) bin_atom<Mir>(a, g, i);
   if (!grid_barrier(bar, a.flags)) return;
-  if (blockIdx.x == 0) scan_cells<kRebuildBlock>(g);
+  scan_cells_grid<kRebuildBlock>(g, 1);
+  if (!grid_barrier(bar, a.flags)) return;
+  scan_cells_grid<kRebuildBlock>(g, 2);
   if (!grid_barrier(bar, a.flags)) return;
   for (int i = tid; i < N; i += nthr) scatter_atom(g, i);
   if (!grid_barrier(bar, a.flags)) return;

@@ -359,6 +359,7 @@ struct Engine final : EngineBase {
       const int need = static_cast<int>(
           std::max<int64_t>((lanes + kRebuildBlock - 1) / kRebuildBlock, 1));
       S->nb_rebuild = std::max(1, std::min(std::min(per_cu, 2) * ncu, need));
+      S->nb_rebuild = std::min(S->nb_rebuild, sched::kMaxScanGroups);
       if (per_cu < 1) S->nb_rebuild = 0;  // cannot guarantee residency: separate kernels
       S->g.nl_split = split && S->nb_rebuild > 0 ? 1 : 0;
     }
@@ -803,9 +804,11 @@ int sched_alloc(tdmd_sched* S) {
   if (static_cast<uint64_t>(N) * msz >= (uint64_t{1} << 32))
     return fail(TDMD_E_INVALID, "scheduler: at most " + std::to_string((uint64_t{1} << 32) / msz - 1) +
                                     " atoms in one scheduler at this precision");
+  // kRecPad spare records past the last atom: the list search reads its
+  // candidates in whole groups of eight from the first of a cell's atoms
   char *M, *M2;
-  if ((rc = dalloc(S, &M, N * msz))) return rc;
-  if ((rc = dalloc(S, &M2, N * msz))) return rc;
+  if ((rc = dalloc(S, &M, (static_cast<size_t>(N) + sched::kRecPad) * msz))) return rc;
+  if ((rc = dalloc(S, &M2, (static_cast<size_t>(N) + sched::kRecPad) * msz))) return rc;
   a.M = M;
   a.M2 = M2;
   if ((rc = dalloc(S, &a.flags, 8))) return rc;
@@ -842,6 +845,7 @@ int sched_alloc(tdmd_sched* S) {
   if ((rc = dalloc(S, &g.sidx, N))) return rc;
   if ((rc = dalloc(S, &g.orig, N))) return rc;
   if ((rc = dalloc(S, &g.done, 4))) return rc;
+  if ((rc = dalloc(S, &g.wsum, sched::kMaxScanGroups))) return rc;
   if ((rc = dalloc(S, &S->d_maxcnt, 4))) return rc;
   if ((rc = dalloc(S, &S->d_rebuilds, 1))) return rc;
   if ((rc = dalloc(S, &S->d_scalars, 16))) return rc;
