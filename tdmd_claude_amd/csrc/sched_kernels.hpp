@@ -235,6 +235,7 @@ struct StepArgs {
   int nblocks;  // grid size (multiple of 8)
   int cap;
   int ntypes;
+  int one_type;  // 1: every atom is of type 1 (the integrator skips types[])
   double dt;
   double skin_half_sq;
   MirrorParams mp;

@@ -75,7 +75,7 @@ template <class R, class Mir>
 __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vscale,
                                            double kick_dt, double f0, double f1, double f2,
                                            void* rec) {
-  const int ti = ntl(&a.types[i]) - 1;
+  const int ti = a.one_type ? 0 : ntl(&a.types[i]) - 1;
   const double mass = a.masses[ti + 1];
   const double factor = kick_dt / (mass * kMvv2e);
   const double vx = vscale * ntl(&a.vx[i]) + factor * f0;
@@ -128,7 +128,7 @@ __device__ __forceinline__ double epilogue(const StepArgs& a, int i, double fx, 
   if (kEpi == kForceOnly) return 0.0;
   // kHalfKick / kKick2Ke: v += (dt/2) F / (m mvv2e)  (half_kick_kernel,
   // device_velocity_verlet.cu:14-32)
-  const int ti = ntl(&a.types[i]) - 1;
+  const int ti = a.one_type ? 0 : ntl(&a.types[i]) - 1;
   const double mass = a.masses[ti + 1];
   const double factor = (0.5 * a.dt) / (mass * kMvv2e);
   const double vx = ntl(&a.vx[i]) + factor * static_cast<double>(f0);

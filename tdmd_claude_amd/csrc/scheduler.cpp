@@ -160,6 +160,16 @@ struct tdmd_sched {
 namespace tdmd_hip {
 namespace {
 
+// Destroys every captured graph (the caller has synchronised the stream).
+void drop_graphs(tdmd_sched* S) {
+  for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
+  S->graphs.clear();
+  S->graph_defs.clear();
+  S->graph_kernels.clear();
+  S->graph_swaps.clear();
+}
+
 template <class R, class Mir>
 struct Engine final : EngineBase {
   tdmd_sched* S;
@@ -1072,8 +1082,7 @@ int sched_setup(tdmd_sched* S, const tdmd_box& box, int natoms, const tdmd_sched
 
 void sched_free(tdmd_sched* S) {
   if (S->stream) (void)hipStreamSynchronize(S->stream);
-  for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
+  drop_graphs(S);
   for (void* p : S->allocs) (void)hipFree(p);
   if (S->a.nbr) (void)hipFree(const_cast<int32_t*>(S->a.nbr));
   if (S->a.nbr16) (void)hipFree(const_cast<uint16_t*>(S->a.nbr16));
@@ -1140,12 +1149,7 @@ int first_build(tdmd_sched* S) {
       S->a.nbrc16 = static_cast<uint16_t*>(p);
     }
     S->a.cap = cap;
-    for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
-    for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
-    S->graphs.clear();
-    S->graph_defs.clear();
-    S->graph_kernels.clear();
-    S->graph_swaps.clear();
+    drop_graphs(S);
   }
   rc = S->eng->fill_pass(s);
   if (rc) return rc;
@@ -1428,12 +1432,7 @@ int tdmd_sched_autotune(tdmd_sched* S, int32_t iters, int32_t out[2]) {
   if (rc) return rc;
   // captured graphs hold the previous kernel choice
   TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
-  for (auto& kv : S->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto& kv : S->graph_defs) (void)hipGraphDestroy(kv.second);
-  S->graphs.clear();
-  S->graph_defs.clear();
-  S->graph_kernels.clear();
-  S->graph_swaps.clear();
+  drop_graphs(S);
   S->cfg.pair_unroll = out[0];
   S->cfg.lds_budget = out[1];
   S->cfg.pair_lanes = S->lanes;
@@ -1490,6 +1489,13 @@ int tdmd_sched_upload(tdmd_sched* S, const double* pos, const double* vel, const
     }
   }
   TDMD_HIP_TRY(hipMemcpy(const_cast<int32_t*>(S->a.types), types, N * 4, hipMemcpyHostToDevice));
+  // captured graphs hold the kernel arguments, this flag among them
+  const int one_type = std::all_of(types, types + N, [](int32_t t) { return t == 1; }) ? 1 : 0;
+  if (one_type != S->a.one_type) {
+    TDMD_HIP_TRY(hipStreamSynchronize(S->stream));
+    drop_graphs(S);
+    S->a.one_type = one_type;
+  }
   std::vector<double> m64(64, 1.0);
   std::copy(masses, masses + n_masses, m64.begin());
   TDMD_HIP_TRY(hipMemcpy(const_cast<double*>(S->a.masses), m64.data(), 64 * 8, hipMemcpyHostToDevice));

@@ -132,6 +132,31 @@ def test_graph_and_direct_launch_are_bitwise_identical(pot):
         assert np.array_equal(a, b)
 
 
+def test_reupload_with_two_types_matches_a_fresh_scheduler():
+    # When every atom is of type 1 the integrator does not read the types
+    # (a flag that captured graphs hold). Re-uploading a two-type, two-mass
+    # state into a scheduler whose graphs were captured for one type must
+    # give what a fresh scheduler gives, bit for bit.
+    s1 = fcc(5)
+    s2 = s1.copy()
+    s2.types = np.where(np.arange(s2.natoms) % 3 == 0, 2, 1).astype(np.int32)
+    s2.masses = np.append(s1.masses, 2.0 * s1.masses[1])
+    say("re-upload with two types")
+    cfg = FastPipelineConfig(precision="mixed", rebuild_policy="displacement")
+    sched = FastPipelineScheduler.for_system(s1, MORSE_CU, cfg)
+    sched.run_until(20)
+    sched.upload_system(s2)
+    sched.run_until(20)
+    got = sched.download()
+    sched.close()
+    fresh = FastPipelineScheduler.for_system(s2, MORSE_CU, cfg)
+    fresh.run_until(20)
+    want = fresh.download()
+    fresh.close()
+    for a, b in zip(got[:3], want[:3]):
+        assert np.array_equal(a, b)
+
+
 def test_download_returns_upload_order():
     s = fcc(5)
     say("upload order")
