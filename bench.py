@@ -132,10 +132,12 @@ def bytes_per_atom(precision: str, mirror_fp64: bool, nbar: float, staged: float
     rows_d = staged * (2 * nbar + 4 * halo) + (1 - staged) * 4 * nbar
     rows_f = staged * (2 * nn + 4 * halo) + (1 - staged) * 4 * nbar
     density = rec + 4 + rows_d + rec  # read M_i, count, rows; write M2_i
-    # NVE force pass: read M2_i, count, type, x, v, x_build, rows; write x, v, M_i
-    force_nve = rec + 4 + 4 + 24 + 24 + xb + rows_f + 24 + 24 + rec
-    # NVT force pass: read M2_i, count, type, v, rows; write F, v
-    force_nvt = rec + 4 + 4 + 24 + rows_f + fb + 24
+    # (one element: the integrator reads no types)
+    # NVE force pass: read M2_i, count, x, v, x_build, rows; write x, v, M_i
+    force_nve = rec + 4 + 24 + 24 + xb + rows_f + 24 + 24 + rec
+    # NVT force pass inside a window: read M2_i, count, v, rows; write F (the
+    # kicked v only enters the KE; the next kick-drift applies that kick)
+    force_nvt = rec + 4 + 24 + rows_f + fb
     return dict(density=density, force_nve=force_nve, force_nvt=force_nvt)
 
 
@@ -745,8 +747,8 @@ def main() -> int:
             "mixed": drift["mixed"], "fp64": drift["fp64"]},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_eam_force_lds<float, MirrorFixed, 3, ...> (force + closing half kick "
-                      "+ KE partials: the force pass of the timed NVT step)",
+            "kernel": "k_eam_force_lds<float, MirrorFixed, 4, ...> (force + KE partials of the "
+                      "half-kicked velocities: the force pass of the timed NVT step)",
             "window": "the timed window's steps, repeated with the pass clock on",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

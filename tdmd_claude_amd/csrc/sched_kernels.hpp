@@ -220,8 +220,13 @@ enum Epilogue : int {
   kForceOnly = 0,    // store F (initial forces, thermo)
   kFullKickDrift = 1,// v += dt F/(m mvv2e); x += dt v (cross-step fused kick)
   kHalfKick = 2,     // v += dt/2 F/(m mvv2e); store F (window end)
-  kKick2Ke = 3       // NVT: v += dt/2 F/m; store F, v; KE partials
+  kKick2Ke = 3,      // NVT: v += dt/2 F/m; store F, v; KE partials
+  kKe = 4            // NVT inside a window: KE partials of v + dt/2 F/m, store F
+                     // only; the next kick-drift applies that half kick itself
 };
+
+// Epilogues that emit per-block KE partials.
+constexpr bool emits_ke(int epi) { return epi == kKick2Ke || epi == kKe; }
 
 // Nose-Hoover chain constants (device_nose_hoover.cu:206-289).
 struct NhcParams {

@@ -71,16 +71,28 @@ __device__ __forceinline__ void nts(T v, T* p) {
   __builtin_nontemporal_store(v, p);
 }
 
-template <class R, class Mir>
+// kKick2: the stored v lacks the previous step's closing half kick (its
+// force pass only summed the KE of the kicked velocity, epilogue kKe); it is
+// applied here first, as v + (dt/2) F / (m mvv2e), the epilogue's expression.
+template <class R, class Mir, bool kKick2 = false>
 __device__ __forceinline__ void kick_drift(const StepArgs& a, int i, double vscale,
                                            double kick_dt, double f0, double f1, double f2,
                                            void* rec) {
   const int ti = a.one_type ? 0 : ntl(&a.types[i]) - 1;
   const double mass = a.masses[ti + 1];
+  double ux = ntl(&a.vx[i]), uy = ntl(&a.vy[i]), uz = ntl(&a.vz[i]);
+  if constexpr (kKick2) {
+    const double half = (0.5 * a.dt) / (mass * kMvv2e);
+    ux = __fma_rn(half, f0, ux);
+    uy = __fma_rn(half, f1, uy);
+    uz = __fma_rn(half, f2, uz);
+  }
+  // explicit fused operations: the same roundings whether the half kick
+  // above was applied here or by the force pass (kick 2 stored)
   const double factor = kick_dt / (mass * kMvv2e);
-  const double vx = vscale * ntl(&a.vx[i]) + factor * f0;
-  const double vy = vscale * ntl(&a.vy[i]) + factor * f1;
-  const double vz = vscale * ntl(&a.vz[i]) + factor * f2;
+  const double vx = __fma_rn(factor, f0, vscale * ux);
+  const double vy = __fma_rn(factor, f1, vscale * uy);
+  const double vz = __fma_rn(factor, f2, vscale * uz);
   const MirrorParams& mp = mp_of<Mir>(a, i);
   const double x = wrap1(ntl(&a.px[i]) + a.dt * vx, mp.lo[0], mp.L[0], mp.periodic[0]);
   const double y = wrap1(ntl(&a.py[i]) + a.dt * vy, mp.lo[1], mp.L[1], mp.periodic[1]);
@@ -131,13 +143,15 @@ __device__ __forceinline__ double epilogue(const StepArgs& a, int i, double fx, 
   const int ti = a.one_type ? 0 : ntl(&a.types[i]) - 1;
   const double mass = a.masses[ti + 1];
   const double factor = (0.5 * a.dt) / (mass * kMvv2e);
-  const double vx = ntl(&a.vx[i]) + factor * static_cast<double>(f0);
-  const double vy = ntl(&a.vy[i]) + factor * static_cast<double>(f1);
-  const double vz = ntl(&a.vz[i]) + factor * static_cast<double>(f2);
-  nts(vx, &a.vx[i]);
-  nts(vy, &a.vy[i]);
-  nts(vz, &a.vz[i]);
-  if (kEpi == kKick2Ke) return 0.5 * mass * kMvv2e * (vx * vx + vy * vy + vz * vz);
+  const double vx = __fma_rn(factor, static_cast<double>(f0), ntl(&a.vx[i]));
+  const double vy = __fma_rn(factor, static_cast<double>(f1), ntl(&a.vy[i]));
+  const double vz = __fma_rn(factor, static_cast<double>(f2), ntl(&a.vz[i]));
+  if (kEpi != kKe) {
+    nts(vx, &a.vx[i]);
+    nts(vy, &a.vy[i]);
+    nts(vz, &a.vz[i]);
+  }
+  if (emits_ke(kEpi)) return 0.5 * mass * kMvv2e * (vx * vx + vy * vy + vz * vz);
   return 0.0;
 }
 
@@ -148,14 +162,15 @@ __device__ __forceinline__ double epilogue(const StepArgs& a, int i, double fx, 
 // Window bootstrap (fused_half_kick_drift) and the NVT first kick: kick by
 // dt/2 with the stored forces, drift, write M. NVT scales v by the chain's
 // pending factor first.
-template <class R, class Mir, bool kNvt, bool kFull>
+template <class R, class Mir, bool kNvt, bool kFull, bool kKick2 = false>
 __global__ void __launch_bounds__(kBlock) k_kick_drift(StepArgs a) {
   const int i = xcd_block(a.nblocks) * kBlock + threadIdx.x;
   if (i >= a.natoms) return;
   const double s = kNvt ? a.nhc[26] : 1.0;
-  kick_drift<R, Mir>(a, i, s, kFull ? a.dt : 0.5 * a.dt, static_cast<double>(ntl(&rp<R>(a.fx)[i])),
-                     static_cast<double>(ntl(&rp<R>(a.fy)[i])), static_cast<double>(ntl(&rp<R>(a.fz)[i])),
-                     a.M);
+  kick_drift<R, Mir, kKick2>(a, i, s, kFull ? a.dt : 0.5 * a.dt,
+                             static_cast<double>(ntl(&rp<R>(a.fx)[i])),
+                             static_cast<double>(ntl(&rp<R>(a.fy)[i])),
+                             static_cast<double>(ntl(&rp<R>(a.fz)[i])), a.M);
 }
 
 // Mirror refresh (after upload or restore): M[i] = {x, type}.
@@ -594,7 +609,7 @@ k_eam_force(StepArgs a, const SplineMeta* __restrict__ meta,
   } else {
     vir = 0.0;
   }
-  if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
+  if (kEnergy || emits_ke(kEpi)) emit_partial(red, a.partials, blk);
   if (kEnergy) emit_partial(vir, a.partials_w, blk);
   pass_clock_end(pclk);
 }
@@ -658,7 +673,7 @@ __global__ void __launch_bounds__(kBlock) k_morse_force(StepArgs a, MorseCoef p)
   } else {
     vir = 0.0;
   }
-  if (kEnergy || kEpi == kKick2Ke) emit_partial(red, a.partials, blk);
+  if (kEnergy || emits_ke(kEpi)) emit_partial(red, a.partials, blk);
   if (kEnergy) emit_partial(vir, a.partials_w, blk);
   pass_clock_end(pclk);
 }

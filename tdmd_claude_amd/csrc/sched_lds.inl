@@ -393,7 +393,7 @@ k_eam_force_lds(StepArgs a, const SplineMeta* __restrict__ meta,
     red = kEnergy ? pe : ke;
     wsum = vir;
   }
-  if (kEnergy || kEpi == kKick2Ke) emit_partial<kBigBlock>(red, a.partials, blk);
+  if (kEnergy || emits_ke(kEpi)) emit_partial<kBigBlock>(red, a.partials, blk);
   if (kEnergy) emit_partial<kBigBlock>(wsum, a.partials_w, blk);
   __syncthreads();  // the next block's halo overwrites this one
   }

@@ -59,6 +59,7 @@ enum Op : char {
   OP_KD_FULL = 'k',     // full kick + drift as its own launch (Morse pass timing)
   OP_NVT_PRE = 'p',     // chain half-step before the step
   OP_KD_NVT = 'q',      // scaled half kick + drift
+  OP_KD_NVT2 = 'Q',     // the previous step's closing half kick, then as 'q'
   OP_REB_FORCED = 'R',  // set flag + rebuild
   OP_REB_COND = 'r',    // rebuild if the device flag is set
   OP_DENS = 'd',        // EAM density pass
@@ -66,6 +67,8 @@ enum Op : char {
   OP_F_ONLY = 'o',      // force, store F
   OP_F_HALF = 'h',      // force + closing half kick
   OP_F_KICK2 = '2',     // force + half kick + KE partials (NVT)
+  OP_F_KE = 'e',        // force + KE partials of the half-kicked v, v not stored
+                        // (NVT inside a window: the next 'Q' applies the kick)
   OP_NVT_POST = 'P',    // chain half-step after the step
   OP_NVT_POST_PRE = 'N',// chain half-step after the step + the next step's
   OP_NVT_FINAL = 'Z'    // apply pending chain scale, commit
@@ -244,7 +247,8 @@ struct Engine final : EngineBase {
         (rc = allow_lds(&k_eam_force_lds<R, Mir, kFullKickDrift, false, kWin, kU, kSt, kIs>,
                         L)) ||
         (rc = allow_lds(&k_eam_force_lds<R, Mir, kHalfKick, false, kWin, kU, kSt, kIs>, L)) ||
-        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU, kSt, kIs>, L)))
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKick2Ke, false, kWin, kU, kSt, kIs>, L)) ||
+        (rc = allow_lds(&k_eam_force_lds<R, Mir, kKe, false, kWin, kU, kSt, kIs>, L)))
       return rc;
     return TDMD_OK;
   }
@@ -731,6 +735,9 @@ struct Engine final : EngineBase {
       case OP_KD_NVT:
         k_kick_drift<R, Mir, true, false><<<nb, kBlock, 0, s>>>(S->a);
         break;
+      case OP_KD_NVT2:
+        k_kick_drift<R, Mir, true, false, true><<<nb, kBlock, 0, s>>>(S->a);
+        break;
       case OP_NVT_PRE:
         k_nhc_pre<<<1, 64, 0, s>>>(S->a.nhc, S->nhcp);
         break;
@@ -768,6 +775,9 @@ struct Engine final : EngineBase {
         break;
       case OP_F_KICK2:
         force<kKick2Ke, false>(s);
+        break;
+      case OP_F_KE:
+        force<kKe, false>(s);
         break;
       default:
         return fail(TDMD_E_STATE, std::string("scheduler: unknown op ") + op);
@@ -1245,8 +1255,10 @@ int run_window(tdmd_sched* S, int n) {
   for (int k = 1; k <= n; ++k) {
     std::string st;
     if (nvt) {
-      if (k == 1) st += static_cast<char>(OP_NVT_PRE);  // later steps: fused into 'N'
-      st += static_cast<char>(OP_KD_NVT);
+      // later steps: the chain's pre half-step is fused into 'N', and the
+      // previous step's closing half kick into this kick-drift
+      if (k == 1) st += static_cast<char>(OP_NVT_PRE);
+      st += static_cast<char>(k == 1 ? OP_KD_NVT : OP_KD_NVT2);
     } else if (k == 1) {
       st += static_cast<char>(OP_BOOT);
     }  // NVE: later steps' kick-drift is the previous force pass's epilogue
@@ -1259,7 +1271,9 @@ int run_window(tdmd_sched* S, int n) {
     }
     if (eam) st += static_cast<char>(OP_DENS);
     if (nvt) {
-      st += static_cast<char>(OP_F_KICK2);
+      // the window's last force pass stores the kicked velocities (the
+      // window end scales them; the next window starts from them)
+      st += static_cast<char>(k < n ? OP_F_KE : OP_F_KICK2);
       st += static_cast<char>(k < n ? OP_NVT_POST_PRE : OP_NVT_POST);
     } else if (k == n) {
       st += static_cast<char>(OP_F_HALF);
@@ -1598,8 +1612,8 @@ int tdmd_sched_interaction_cutoff(const tdmd_sched* S, double* rc) {
 int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
   // A window of `steps` steps with HIP events around each pass of the
   // steady-state step: EAM NVE -> [density, force + kick-drift], EAM NVT ->
-  // [density, force + half kick + KE partials] (the chain half-steps and the
-  // scaled kick-drift run between them, untimed), Morse -> [kick-drift,
+  // [density, force + KE partials] (the chain half-steps and the scaled
+  // kick-drift run between them, untimed), Morse -> [kick-drift,
   // force]. No rebuild test inside the window.
   if (!S || !ms_out || steps < 2) return fail(TDMD_E_INVALID, "sched_time_passes: bad args");
   if (!S->uploaded) return fail(TDMD_E_STATE, "sched_time_passes: upload() first");
@@ -1610,7 +1624,7 @@ int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
   if (!S->initial_forces && (rc = compute_initial_forces(S))) return rc;
   hipStream_t s = S->stream;
   const char a_op = eam ? OP_DENS : OP_KD_FULL;
-  const char b_op = nvt ? OP_F_KICK2 : eam ? OP_F_FULL : OP_F_ONLY;
+  const char b_op = nvt ? OP_F_KE : eam ? OP_F_FULL : OP_F_ONLY;
   // Events are stamped by the kernel dispatches themselves
   // (hipExtLaunchKernel), so each interval is one kernel's execution.
   std::vector<hipEvent_t> ev(static_cast<size_t>(4 * steps));
@@ -1622,7 +1636,7 @@ int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
     if (!eam && (rc = S->eng->launch(OP_F_ONLY, s))) return rc;
   }
   for (int k = 0; k < steps - 1; ++k) {
-    if (nvt && (rc = S->eng->launch(OP_KD_NVT, s))) return rc;
+    if (nvt && (rc = S->eng->launch(k == 0 ? OP_KD_NVT : OP_KD_NVT2, s))) return rc;
     S->eng->time_next(ev[4 * k], ev[4 * k + 1]);
     if ((rc = S->eng->launch(a_op, s))) return rc;
     S->eng->time_next(ev[4 * k + 2], ev[4 * k + 3]);
@@ -1631,7 +1645,7 @@ int tdmd_sched_time_passes(tdmd_sched* S, int32_t steps, float* ms_out) {
     if (nvt && (rc = S->eng->launch(OP_NVT_POST_PRE, s))) return rc;
   }
   if (nvt) {
-    for (char op : {OP_KD_NVT, OP_DENS, OP_F_KICK2, OP_NVT_POST, OP_NVT_FINAL})
+    for (char op : {OP_KD_NVT2, OP_DENS, OP_F_KICK2, OP_NVT_POST, OP_NVT_FINAL})
       if ((rc = S->eng->launch(op, s))) return rc;
   } else {
     if (eam && (rc = S->eng->launch(OP_DENS, s))) return rc;
