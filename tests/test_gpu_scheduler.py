@@ -570,16 +570,19 @@ def test_cell_size_tuner_agrees_to_rounding():
     assert force_rel_err(out[1][2], out[0][2]) < 1e-10
 
 
-@pytest.mark.parametrize("mirror_fp64", [False, True])
-def test_list_counts_match_oracle_at_the_cutoffs(mirror_fp64):
+@pytest.mark.parametrize("mirror_fp64,lz", [(False, 26.0), (True, 26.0), (False, 29.5)])
+def test_list_counts_match_oracle_at_the_cutoffs(mirror_fp64, lz):
     # The list search decides most candidates from the 32-bit fixed-point
     # records in FP32, with the exact FP64 test inside a 1e-5 relative band
     # around r_list^2 and rc^2. Pairs placed 1e-7 from r_list and from rc
     # (inside the band) and 3e-5 from them (outside it): the rows' total,
     # near total and longest row must be the oracle's exact lists, at the
     # first build (one lane per atom) and after a fused rebuild (32 lanes).
+    # A cubic box takes the prefilter in units^2 (one unit on all axes); a
+    # box longer in z has a larger unit there and takes the per-axis scaling.
     rng = np.random.Generator(np.random.PCG64(17))
-    L, rc, skin = 26.0, 6.0, 1.0
+    rc, skin = 6.0, 1.0
+    L = np.array([26.0, 26.0, lz])
     r_list = rc + skin
     base = rng.uniform(0.0, L, size=(300, 3))
     extra = []
@@ -591,7 +594,7 @@ def test_list_counts_match_oracle_at_the_cutoffs(mirror_fp64):
             u /= np.linalg.norm(u)
             extra += [p, np.mod(p + d * u, L)]
     pos = np.concatenate([base, np.array(extra)])
-    s = I.System(lo=np.zeros(3), hi=np.full(3, L), pos=pos, vel=np.zeros_like(pos),
+    s = I.System(lo=np.zeros(3), hi=L.copy(), pos=pos, vel=np.zeros_like(pos),
                  types=np.ones(len(pos), dtype=np.int32),
                  ids=np.arange(1, len(pos) + 1, dtype=np.int32), masses=np.array([1.0, 63.546]))
     say("list counts at the cutoffs, mirror_fp64", mirror_fp64, s.natoms)
