@@ -1547,6 +1547,18 @@ __device__ __forceinline__ uint32_t halo_pos(const HaloChunk* ent, int j) {
   return static_cast<uint32_t>(e.y + __popcll(m & below));
 }
 
+// One row slot j into the thread's current run (chunk cc, mask m); a slot
+// in another chunk ORs the run into the set first (ok turns false when the
+// set is full).
+__device__ __forceinline__ void halo_merge(int j, int& cc, uint64_t& m, bool& ok, HaloLds& L) {
+  if ((j >> 6) != cc) {
+    ok = ok && chunk_or(L.ent, cc, m, &L.nc);
+    cc = j >> 6;
+    m = 0;
+  }
+  m |= uint64_t{1} << (j & 63);
+}
+
 // Halos of blocks blockIdx.x, blockIdx.x + gridDim.x, ... (< nblocks), by a
 // workgroup of kHaloBlock threads; its LDS is `L`.
 __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, HaloLds& L) {
@@ -1577,22 +1589,19 @@ __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, Halo
       int cc = i >> 6;
       uint64_t m = uint64_t{1} << (i & 63);
       bool ok = true;
+      // (the next quad is loaded by index, the last one reloading itself,
+      // rather than selected between a load and a copy: a select of loaded
+      // vectors kept them in scratch memory)
       const int nq = (a.cnt[i] + 3) >> 2;
-      int4 cur = nq > 0 ? q4[ell(i, 0, cap) >> 2] : make_int4(i, i, i, i);
+      int4 cur = make_int4(i, i, i, i);
+      if (nq > 0) cur = q4[ell(i, 0, cap) >> 2];
       for (int q = 0; q < nq && ok; ++q) {
-        const int4 nxt = q + 1 < nq ? q4[ell(i, 4 * (q + 1), cap) >> 2] : cur;
-        auto merge = [&](int j) {
-          if ((j >> 6) != cc) {
-            ok = ok && chunk_or(L.ent, cc, m, &L.nc);
-            cc = j >> 6;
-            m = 0;
-          }
-          m |= uint64_t{1} << (j & 63);
-        };
-        merge(cur.x);
-        merge(cur.y);
-        merge(cur.z);
-        merge(cur.w);
+        const int qn = q + 1 < nq ? q + 1 : q;
+        const int4 nxt = q4[ell(i, 4 * qn, cap) >> 2];
+        halo_merge(cur.x, cc, m, ok, L);
+        halo_merge(cur.y, cc, m, ok, L);
+        halo_merge(cur.z, cc, m, ok, L);
+        halo_merge(cur.w, cc, m, ok, L);
         cur = nxt;
       }
       if (ok) ok = chunk_or(L.ent, cc, m, &L.nc);
@@ -1680,9 +1689,11 @@ __device__ __forceinline__ void build_halos(const StepArgs& a, int nblocks, Halo
       // the padded row in quads; a quad's four 16-bit positions are
       // adjacent in the 16-bit layout (eight slots per 16 bytes)
       const int mq = ((a.cnt[i] + 7) & ~7) >> 2;
-      int4 cur = mq > 0 ? q4[ell(i, 0, cap) >> 2] : make_int4(i, i, i, i);
+      int4 cur = make_int4(i, i, i, i);
+      if (mq > 0) cur = q4[ell(i, 0, cap) >> 2];
       for (int q = 0; q < mq; ++q) {
-        const int4 nxt = q + 1 < mq ? q4[ell(i, 4 * (q + 1), cap) >> 2] : cur;
+        const int qn = q + 1 < mq ? q + 1 : q;
+        const int4 nxt = q4[ell(i, 4 * qn, cap) >> 2];
         const uint32_t p0 = halo_pos(L.ent, cur.x);
         const uint32_t p1 = halo_pos(L.ent, cur.y);
         const uint32_t p2 = halo_pos(L.ent, cur.z);
